@@ -1,0 +1,133 @@
+/*
+ * libcuda-crypt.so — C ABI of the MI355X (gfx950) build.
+ *
+ * Drop-in for the exports of solana-perf-libs' libcuda-crypt.so that
+ * solana_perf::perf_libs binds.  Every declaration below cites the reference
+ * interface it replaces (paths relative to the reference tree).  The packet
+ * layout is unchanged: packets are the concatenation of
+ * elems[i].elems[0 .. elems[i].num * message_size), and every offset is a
+ * byte offset into that concatenation.
+ *
+ * Entry points after "Extensions" have no counterpart in the reference.
+ */
+#ifndef SPL_CUDA_CRYPT_H
+#define SPL_CUDA_CRYPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHA512_SIZE 64
+#define PUB_KEY_SIZE 32
+#define PRIV_KEY_SIZE 64
+#define SEED_SIZE 32
+#define SCALAR_SIZE 32
+#define SIG_SIZE 64
+
+/* src/cuda-ecc-ed25519/ed25519.h:35-38 */
+typedef struct {
+    uint8_t *elems;
+    uint32_t num;
+} gpu_Elems;
+
+/* ed25519.h:25, seed.cu:12-41 — 32 bytes from /dev/urandom; 0 = ok. */
+int ed25519_create_seed(unsigned char *seed);
+
+/* ed25519.h:40, keypair.cu:6-16 — private key is the 64-byte expanded key
+ * clamp(SHA-512(seed)[0:32]) || SHA-512(seed)[32:64]. */
+void ed25519_create_keypair(unsigned char *public_key, unsigned char *private_key,
+                            const unsigned char *seed);
+
+/* ed25519.h:41, sign.cu:40-46 — host, single signature. */
+void ed25519_sign(unsigned char *signature, const unsigned char *message, size_t message_len,
+                  const unsigned char *public_key, const unsigned char *private_key);
+
+/* ed25519.h:43-53, sign.cu:76-145 — GPU batch sign; signatures_out[64*i].
+ * private_key_offsets point at 64-byte expanded private keys in the packets. */
+void ed25519_sign_many(const gpu_Elems *elems, uint32_t num_elems, uint32_t message_size,
+                       uint32_t total_packets, uint32_t total_signatures,
+                       const uint32_t *message_lens, const uint32_t *public_key_offsets,
+                       const uint32_t *private_key_offsets, const uint32_t *message_start_offsets,
+                       uint8_t *signatures_out, uint8_t use_non_default_stream);
+
+/* ed25519.h:55, verify.cu:154-174 — host, single signature; 1 = valid. */
+int ed25519_verify(const unsigned char *signature, const unsigned char *message,
+                   uint32_t message_len, const unsigned char *public_key);
+
+/* ed25519.h:57-67, verify.cu:240-340 — GPU batch verify; out[i] = 1 valid,
+ * 0 invalid.  Synchronous.  total_packets == 0 returns without touching out.
+ * (The reference header names the 8th parameter private_key_offsets; its
+ * definition, like this one, uses it as the signature offsets.)  On a device
+ * error every out[i] is written 0 and the error is printed; the call never
+ * aborts the process. */
+void ed25519_verify_many(const gpu_Elems *elems, uint32_t num_elems, uint32_t message_size,
+                         uint32_t total_packets, uint32_t total_signatures,
+                         const uint32_t *message_lens, const uint32_t *public_key_offsets,
+                         const uint32_t *signature_offsets, const uint32_t *message_start_offsets,
+                         uint8_t *out, uint8_t use_non_default_stream);
+
+/* ed25519.h:71, verify.cu:234-238 */
+void ed25519_set_verbose(bool val);
+
+/* ed25519.h:73, verify.cu:342-347 */
+const char *ed25519_license(void);
+
+/* ed25519.h:74, gpu_ctx.cu:38-44 — true iff at least one GPU is usable. */
+bool ed25519_init(void);
+
+/* ed25519.h:76-77, verify.cu:349-355 — hipHostRegister/hipHostUnregister;
+ * returns the runtime error code, 0 = success. */
+int cuda_host_register(void *ptr, size_t size, unsigned int flags);
+int cuda_host_unregister(void *ptr);
+
+/* ed25519.h:79, verify.cu:60-85 — 0 when the scalar passes the reference's
+ * canonicity gate (top nibble clear: copied to out_scalar). */
+int ed25519_get_checked_scalar(unsigned char *out_scalar, const unsigned char *in_scalar);
+
+/* ed25519.h:81, verify.cu:88-107 — 0 when the encoding decodes and is not of
+ * order <= 4; 1 otherwise. */
+int ed25519_check_packed_ge_small_order(const unsigned char *packed_group_element);
+
+/* gpu_ctx.h:39, gpu_ctx.cu:151-167 — releases the cached device buffers. */
+void ed25519_free_gpu_mem(void);
+
+/* src/cuda-poh-verify/poh_verify.cu:76-78 */
+void poh_verify_many_set_verbose(bool val);
+
+/* poh_verify.cu:80-144 — hashes[32*i..] <- SHA-256^num_hashes_arr[i], in place.
+ * Returns 0 on success, 1 when no GPU is available (hashes untouched). */
+int poh_verify_many(uint8_t *hashes, const uint64_t *num_hashes_arr, size_t num_elems,
+                    uint8_t use_non_default_stream);
+
+/* ------------------------------------------------------------------------
+ * Extensions: device-resident forms (no counterpart in the reference).
+ * All pointers are device pointers on the current HIP device; `stream` is a
+ * hipStream_t (NULL = null stream).  The calls only enqueue work.
+ * Return 0 on success, 1 when no GPU is usable, 2 on bad arguments, 3 on a
+ * launch failure. */
+int ed25519_verify_many_device(const uint8_t *d_packets, uint64_t packets_size,
+                               const uint32_t *d_message_lens, const uint32_t *d_public_key_offsets,
+                               const uint32_t *d_signature_offsets,
+                               const uint32_t *d_message_start_offsets, uint32_t total_signatures,
+                               uint8_t *d_out, void *stream);
+
+int ed25519_sign_many_device(const uint8_t *d_packets, uint64_t packets_size,
+                             const uint32_t *d_message_lens, const uint32_t *d_public_key_offsets,
+                             const uint32_t *d_private_key_offsets,
+                             const uint32_t *d_message_start_offsets, uint32_t total_signatures,
+                             uint8_t *d_signatures_out, void *stream);
+
+int poh_verify_many_device(uint8_t *d_hashes, const uint64_t *d_num_hashes, size_t num_elems,
+                           void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPL_CUDA_CRYPT_H */

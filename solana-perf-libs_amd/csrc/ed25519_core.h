@@ -1,0 +1,260 @@
+// Ed25519 verify / sign / keypair cores shared by the gfx950 kernels and the
+// exported host helpers.
+//
+// Acceptance rule of the reference's verify path
+// (src/cuda-ecc-ed25519/verify.cu:109-174, ge.cu:60-222):
+//   1. S = sig[32..64): reject when its top nibble is set (S >= 2^252);
+//   2. R = sig[0..32): must decode and not be "small order";
+//   3. A = public key: must decode and not be "small order";
+//   4. h = SHA-512(R || A || M) mod L;
+//   5. accept iff encode([h](-A) + [S]B) == R (byte comparison).
+// "Small order" is the reference's raw-limb identity test after three
+// doublings; it fires exactly when the encoding decodes and y mod p is 0, 1 or
+// p-1 (y_low_order).  Because encode() only produces canonical encodings of
+// curve points, step 5 implies that R decodes, and for a canonical R the
+// small-order test reduces to y_low_order(R).  So step 2 is evaluated as
+// "not y_low_order(R)" without decompressing R: same mask, one square root
+// (about 260 field operations) less per signature.
+#pragma once
+#include "ge25519.h"
+#include "sc25519.h"
+#include "sha2.h"
+
+namespace spl {
+
+struct ed25519_tables {
+    curve_consts k;
+    ge_precomp Bi[8];          // (2j+1) B, j = 0..7
+    ge_precomp base[32][8];    // (j+1) 256^i B
+};
+
+// ------------------------------------------------------------ message reader
+
+// Little-endian 32-bit word m of "M || 0x80 || 0 ...": the message bytes
+// 4m..4m+3 followed by the SHA-2 padding byte at position len.
+struct MsgReader {
+    const uint8_t *p;   // first message byte
+    uint32_t len;
+
+    SPL_HD uint32_t word(uint32_t m) const {
+        uint32_t j = 4u * m;
+        if (j > len) return 0u;
+        if (j == len) return 0x80u;
+        uint32_t w;
+#if defined(__HIP_DEVICE_COMPILE__)
+        uintptr_t addr = (uintptr_t)(p + j);
+        const uint32_t *a = (const uint32_t *)(addr & ~(uintptr_t)3);
+        uint32_t sh = (uint32_t)(addr & 3);
+        uint32_t lo = a[0];
+        uint32_t hi = (sh != 0 && j + 4u - sh < len) ? a[1] : 0u;
+        w = __builtin_amdgcn_alignbyte(hi, lo, sh);
+#else
+        w = 0;
+        for (uint32_t b = 0; b < 4 && j + b < len; b++) w |= (uint32_t)p[j + b] << (8 * b);
+#endif
+        uint32_t rem = len - j;
+        if (rem < 4u) {
+            uint32_t keep = (1u << (8u * rem)) - 1u;
+            w = (w & keep) | (0x80u << (8u * rem));
+        }
+        return w;
+    }
+};
+
+// n little-endian words from an arbitrary byte address (inside one buffer).
+SPL_HD void load_words(uint32_t *out, const uint8_t *src, int n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uintptr_t addr = (uintptr_t)src;
+    const uint32_t *a = (const uint32_t *)(addr & ~(uintptr_t)3);
+    uint32_t sh = (uint32_t)(addr & 3);
+    uint32_t prev = a[0];
+    for (int i = 0; i < n; i++) {
+        // the dword after the field is read only when the field straddles it
+        uint32_t next = (i + 1 < n || sh != 0) ? a[i + 1] : 0u;
+        out[i] = __builtin_amdgcn_alignbyte(next, prev, sh);
+        prev = next;
+    }
+#else
+    for (int i = 0; i < n; i++)
+        out[i] = (uint32_t)src[4 * i] | ((uint32_t)src[4 * i + 1] << 8) |
+                 ((uint32_t)src[4 * i + 2] << 16) | ((uint32_t)src[4 * i + 3] << 24);
+#endif
+}
+
+// SHA-512 over prefix (npw little-endian words, npw even) || M.
+SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw,
+                            const MsgReader &msg) {
+    sha512_init(st);
+    uint64_t total = 4ull * npw + msg.len;
+    uint32_t nblocks = (uint32_t)((total + 17 + 127) / 128);
+    for (uint32_t b = 0; b < nblocks; b++) {
+        uint64_t w[16];
+        for (int t = 0; t < 16; t++) {
+            uint32_t k = 32u * b + 2u * t;   // stream word index
+            uint32_t s0 = k < npw ? prefix[k] : msg.word(k - npw);
+            uint32_t s1 = k + 1 < npw ? prefix[k + 1] : msg.word(k + 1 - npw);
+            w[t] = ((uint64_t)bswap32(s0) << 32) | bswap32(s1);
+        }
+        if (b == nblocks - 1) {
+            w[14] = 0;
+            w[15] = total * 8;
+        }
+        sha512_block(st, w);
+    }
+}
+
+// ------------------------------------------------------------ verify
+
+// Returns 1 when the signature verifies under the rule above.  Ai is caller
+// storage for the 8 cached odd multiples of -A.
+SPL_HD int verify_one(const uint32_t sig[16], const uint32_t pk[8], const MsgReader &msg,
+                      const ed25519_tables &T, ge_cached *Ai) {
+    if (sig[15] & 0xf0000000u) return 0;           // S >= 2^252
+    if (y_low_order(sig)) return 0;                // R of order <= 4
+    uint32_t pre[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) { pre[i] = sig[i]; pre[8 + i] = pk[i]; }
+    uint64_t st[8];
+    sha512_prefixed(st, pre, 16, msg);
+    uint32_t hw[16], h[8];
+    sha512_digest_le_words(hw, st);
+    sc_reduce512(h, hw);
+
+    ge_p3 A;
+    if (ge_decode_neg(A, pk, T.k) != 0) return 0;
+    if (y_low_order(pk)) return 0;
+
+    ge_p1p1 t;
+    ge_p3 A2, u;
+    ge_p3_to_cached(Ai[0], A, T.k.d2);
+    ge_p3_dbl(t, A);
+    ge_p1p1_to_p3(A2, t);
+    for (int j = 1; j < 8; j++) {
+        ge_add_cached(t, A2, Ai[j - 1], 0);
+        ge_p1p1_to_p3(u, t);
+        ge_p3_to_cached(Ai[j], u, T.k.d2);
+    }
+
+    int8_t as[256], bs[256];
+    sc_slide(as, h);
+    sc_slide(bs, sig + 8);
+    ge_p2 r;
+    ge_p2_0(r);
+    int i = 255;
+    while (i >= 0 && !as[i] && !bs[i]) i--;
+    for (; i >= 0; i--) {
+        ge_p2_dbl(t, r);
+        int da = as[i], db = bs[i];
+        if (da) {
+            ge_p1p1_to_p3(u, t);
+            ge_add_cached(t, u, Ai[(da > 0 ? da : -da) >> 1], da < 0);
+        }
+        if (db) {
+            ge_p1p1_to_p3(u, t);
+            ge_add_precomp(t, u, T.Bi[(db > 0 ? db : -db) >> 1], db < 0);
+        }
+        ge_p1p1_to_p2(r, t);
+    }
+    uint32_t enc[8];
+    ge_encode(enc, r.X, r.Y, r.Z);
+    uint32_t diff = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) diff |= enc[j] ^ sig[j];
+    return diff == 0;
+}
+
+// ------------------------------------------------------------ fixed-base mult
+
+// h = a B for a 256-bit scalar with a < 2^255 (signed radix-16, constant-time
+// table scans: the scalar is secret when signing).  Same digit recoding and
+// table shape as the reference's ge_scalarmult_base (ge.cu:452-500).
+SPL_HD void ge_scalarmult_base(ge_p3 &h, const uint32_t a[8], const ed25519_tables &T) {
+    int8_t e[64];
+    for (int i = 0; i < 64; i++) e[i] = (int8_t)((a[i >> 3] >> (4 * (i & 7))) & 15);
+    int carry = 0;
+    for (int i = 0; i < 63; i++) {
+        int v = e[i] + carry;
+        carry = (v + 8) >> 4;
+        e[i] = (int8_t)(v - (carry << 4));
+    }
+    e[63] = (int8_t)(e[63] + carry);
+    ge_p3_0(h);
+    ge_p1p1 r;
+    for (int pass = 1; pass >= 0; pass--) {
+        for (int i = pass; i < 64; i += 2) {
+            int d = e[i];
+            uint32_t neg = (uint32_t)d >> 31;             // 1 when d < 0
+            uint32_t ad = (uint32_t)(d < 0 ? -d : d);      // 0..8
+            ge_precomp sel;
+            fe_set(sel.yplusx, 1); fe_set(sel.yminusx, 1); fe_set(sel.xy2d, 0);
+            for (uint32_t j = 0; j < 8; j++) {
+                uint32_t hit = (ad == j + 1);
+                const ge_precomp &c = T.base[i >> 1][j];
+                fe_cmov(sel.yplusx, c.yplusx, hit);
+                fe_cmov(sel.yminusx, c.yminusx, hit);
+                fe_cmov(sel.xy2d, c.xy2d, hit);
+            }
+            ge_precomp ng;
+            ng.yplusx = sel.yminusx;
+            ng.yminusx = sel.yplusx;
+            fe_neg(ng.xy2d, sel.xy2d);
+            fe_cmov(sel.yplusx, ng.yplusx, neg);
+            fe_cmov(sel.yminusx, ng.yminusx, neg);
+            fe_cmov(sel.xy2d, ng.xy2d, neg);
+            ge_madd_ct(r, h, sel);
+            ge_p1p1_to_p3(h, r);
+        }
+        if (pass == 1) {
+            ge_p2 s;
+            ge_p3_dbl(r, h);
+            ge_p1p1_to_p2(s, r);
+            ge_p2_dbl(r, s);
+            ge_p1p1_to_p2(s, r);
+            ge_p2_dbl(r, s);
+            ge_p1p1_to_p2(s, r);
+            ge_p2_dbl(r, s);
+            ge_p1p1_to_p3(h, r);
+        }
+    }
+}
+
+// ------------------------------------------------------------ sign, keypair
+
+// RFC 8032 signature with the 64-byte expanded private key
+// (clamped scalar || prefix), the reference's composition (sign.cu:9-38).
+SPL_HD void sign_one(uint32_t sig[16], const uint32_t pk[8], const uint32_t sk[16],
+                     const MsgReader &msg, const ed25519_tables &T) {
+    uint64_t st[8];
+    uint32_t dw[16], r[8], hram[8];
+    sha512_prefixed(st, sk + 8, 8, msg);
+    sha512_digest_le_words(dw, st);
+    sc_reduce512(r, dw);
+    ge_p3 R;
+    ge_scalarmult_base(R, r, T);
+    ge_encode(sig, R.X, R.Y, R.Z);
+    uint32_t pre[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) { pre[i] = sig[i]; pre[8 + i] = pk[i]; }
+    sha512_prefixed(st, pre, 16, msg);
+    sha512_digest_le_words(dw, st);
+    sc_reduce512(hram, dw);
+    sc_muladd(sig + 8, hram, sk, r);
+}
+
+// ed25519_create_keypair (keypair.cu:6-16): sk = clamp(SHA-512(seed)[0:32]) ||
+// SHA-512(seed)[32:64], pk = encode(sk_scalar B).
+SPL_HD void keypair_from_seed(uint32_t pk[8], uint32_t sk[16], const uint32_t seed[8],
+                              const ed25519_tables &T) {
+    MsgReader none{nullptr, 0};
+    uint64_t st[8];
+    sha512_prefixed(st, seed, 8, none);
+    sha512_digest_le_words(sk, st);
+    sk[0] &= ~7u;
+    sk[7] &= 0x3fffffffu;
+    sk[7] |= 0x40000000u;
+    ge_p3 A;
+    ge_scalarmult_base(A, sk, T);
+    ge_encode(pk, A.X, A.Y, A.Z);
+}
+
+}  // namespace spl

@@ -1,0 +1,93 @@
+// Device context pool and host-side plumbing for libcuda-crypt.so on gfx950.
+//
+// Mirrors the reference's pool semantics (src/cuda-ecc-ed25519/gpu_ctx.cu:6-167,
+// src/cuda-poh-verify/poh_verify.cu:29-72): up to 8 devices x 8 queues, picked
+// round-robin per call, one mutex per queue held for the whole call, grow-only
+// device buffers.  Differences, by design:
+//   - errors are reported (stderr + a return code / zeroed mask), never abort;
+//   - packets and offset tables are packed into one pinned staging buffer and
+//     sent with a single H2D copy instead of one copy per gpu_Elems;
+//   - each device holds one copy of the precomputed base-point tables.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "ed25519_core.h"
+
+namespace spl {
+
+extern bool g_verbose;
+
+#define SPL_LOG(...)                      \
+    do {                                  \
+        if (::spl::g_verbose) {           \
+            fprintf(stderr, __VA_ARGS__); \
+        }                                 \
+    } while (0)
+
+// Evaluate a HIP call; on failure print it and jump to the caller's `fail`.
+#define SPL_HIP_TRY(expr)                                                       \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            fprintf(stderr, "libcuda-crypt: %s failed: %s (%s:%d)\n", #expr,   \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                 \
+            goto fail;                                                          \
+        }                                                                       \
+    } while (0)
+
+constexpr int kMaxDevices = 8;
+constexpr int kQueuesPerDevice = 8;
+
+// A grow-only buffer; device or pinned host memory.
+struct Buf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    bool host = false;
+    hipError_t reserve(size_t bytes);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct QueueCtx {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    Buf d_packets, d_offsets, d_out, d_scratch, d_hashes, d_counts;
+    Buf h_stage;     // pinned host staging
+};
+
+struct DeviceCtx {
+    int id = -1;
+    ed25519_tables *d_tables = nullptr;
+    QueueCtx q[kQueuesPerDevice];
+};
+
+// Host copy of the tables (computed once, on first use).
+const ed25519_tables &host_tables();
+
+// Lazily initialises the pool; returns the number of usable devices (0 when
+// there is no GPU or HIP fails to initialise).
+int pool_init();
+
+// Picks (device, queue) round-robin, locks the queue and makes its device
+// current.  Returns nullptr when there is no device.  Release with
+// release_queue().
+struct Lease {
+    DeviceCtx *dev = nullptr;
+    QueueCtx *q = nullptr;
+};
+Lease acquire_queue();
+void release_queue(const Lease &l);
+
+// Device copy of the tables for HIP device `hip_device` (nullptr if that
+// device is not in the pool).
+const ed25519_tables *device_tables(int hip_device);
+
+// Frees every cached device/pinned buffer (ed25519_free_gpu_mem).
+void pool_free_buffers();
+
+}  // namespace spl

@@ -1,0 +1,214 @@
+// Device context pool, precomputed tables, pinned staging.
+#include "runtime.h"
+
+#include <string.h>
+
+namespace spl {
+
+bool g_verbose = false;
+
+hipError_t Buf::reserve(size_t bytes) {
+    if (bytes <= cap && ptr) return hipSuccess;
+    release();
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = host ? hipHostMalloc(&ptr, want, hipHostMallocDefault)
+                        : hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        cap = 0;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void Buf::release() {
+    if (ptr) {
+        if (host) (void)hipHostFree(ptr);
+        else (void)hipFree(ptr);
+    }
+    ptr = nullptr;
+    cap = 0;
+}
+
+// ------------------------------------------------------------ tables
+
+static ed25519_tables g_tables;
+static std::once_flag g_tables_once;
+
+static void p3_add(ge_p3 &r, const ge_p3 &p, const ge_p3 &q, const fe &d2) {
+    ge_cached c;
+    ge_p1p1 t;
+    ge_p3_to_cached(c, q, d2);
+    ge_add_cached(t, p, c, 0);
+    ge_p1p1_to_p3(r, t);
+}
+
+static void p3_to_precomp(ge_precomp &r, const ge_p3 &p, const fe &d2) {
+    fe zi, x, y;
+    fe_invert(zi, p.Z);
+    fe_mul(x, p.X, zi);
+    fe_mul(y, p.Y, zi);
+    fe_canon(x, x);
+    fe_canon(y, y);
+    fe_add(r.yplusx, y, x);
+    fe_sub(r.yminusx, y, x);
+    fe_mul(r.xy2d, x, y);
+    fe_mul(r.xy2d, r.xy2d, d2);
+    fe_canon(r.yplusx, r.yplusx);
+    fe_canon(r.yminusx, r.yminusx);
+    fe_canon(r.xy2d, r.xy2d);
+}
+
+static void compute_tables(ed25519_tables &T) {
+    fe a, b;
+    // d = -121665 / 121666
+    fe_set(a, 121666);
+    fe_invert(b, a);
+    fe_set(a, 121665);
+    fe_mul(b, b, a);
+    fe_neg(T.k.d, b);
+    fe_canon(T.k.d, T.k.d);
+    fe_add(T.k.d2, T.k.d, T.k.d);
+    fe_canon(T.k.d2, T.k.d2);
+    // sqrt(-1) = 2^((p-1)/4) = (2^((p-5)/8))^2 * 2
+    fe_set(a, 2);
+    fe_pow22523(b, a);
+    fe_sq(b, b);
+    fe_mul(T.k.sqrtm1, b, a);
+    fe_canon(T.k.sqrtm1, T.k.sqrtm1);
+    // B: y = 4/5 with x even (encoding 0x58 0x66 ... 0x66)
+    uint32_t enc[8];
+    for (int i = 0; i < 8; i++) enc[i] = 0x66666666u;
+    enc[0] = 0x66666658u;
+    ge_p3 B;
+    ge_decode_neg(B, enc, T.k);
+    fe_neg(B.X, B.X);
+    fe_neg(B.T, B.T);
+    ge_p3 B2, acc;
+    ge_p1p1 t;
+    ge_p3_dbl(t, B);
+    ge_p1p1_to_p3(B2, t);
+    acc = B;
+    for (int j = 0; j < 8; j++) {
+        p3_to_precomp(T.Bi[j], acc, T.k.d2);
+        p3_add(acc, acc, B2, T.k.d2);
+    }
+    ge_p3 row = B;
+    for (int i = 0; i < 32; i++) {
+        ge_p3 m = row;
+        for (int j = 0; j < 8; j++) {
+            p3_to_precomp(T.base[i][j], m, T.k.d2);
+            p3_add(m, m, row, T.k.d2);
+        }
+        for (int k = 0; k < 8; k++) {
+            ge_p3_dbl(t, row);
+            ge_p1p1_to_p3(row, t);
+        }
+    }
+}
+
+const ed25519_tables &host_tables() {
+    std::call_once(g_tables_once, [] { compute_tables(g_tables); });
+    return g_tables;
+}
+
+// ------------------------------------------------------------ pool
+
+static std::mutex g_pool_mu;
+static int g_num_devices = -1;
+static DeviceCtx g_dev[kMaxDevices];
+static uint32_t g_next_dev = 0;
+static uint32_t g_next_q[kMaxDevices] = {0};
+
+static int pool_init_locked() {
+    if (g_num_devices >= 0) return g_num_devices;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        g_num_devices = 0;
+        SPL_LOG("libcuda-crypt: no HIP device\n");
+        return 0;
+    }
+    if (n > kMaxDevices) n = kMaxDevices;
+    const ed25519_tables &T = host_tables();
+    int ok = 0;
+    for (int d = 0; d < n; d++) {
+        DeviceCtx &dc = g_dev[ok];
+        dc.id = d;
+        SPL_HIP_TRY(hipSetDevice(d));
+        SPL_HIP_TRY(hipMalloc(&dc.d_tables, sizeof(ed25519_tables)));
+        SPL_HIP_TRY(hipMemcpy(dc.d_tables, &T, sizeof(ed25519_tables), hipMemcpyHostToDevice));
+        for (int q = 0; q < kQueuesPerDevice; q++) {
+            SPL_HIP_TRY(hipStreamCreate(&dc.q[q].stream));
+            dc.q[q].h_stage.host = true;
+        }
+        ok++;
+        continue;
+    fail:
+        fprintf(stderr, "libcuda-crypt: device %d unusable, skipped\n", d);
+    }
+    g_num_devices = ok;
+    SPL_LOG("libcuda-crypt: %d device(s)\n", ok);
+    return ok;
+}
+
+int pool_init() {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    return pool_init_locked();
+}
+
+Lease acquire_queue() {
+    Lease l;
+    {
+        std::lock_guard<std::mutex> g(g_pool_mu);
+        int n = pool_init_locked();
+        if (n <= 0) return l;
+        uint32_t d = g_next_dev;
+        g_next_dev = (g_next_dev + 1) % (uint32_t)n;
+        uint32_t q = g_next_q[d];
+        g_next_q[d] = (g_next_q[d] + 1) % kQueuesPerDevice;
+        l.dev = &g_dev[d];
+        l.q = &g_dev[d].q[q];
+    }
+    l.q->mu.lock();
+    if (hipSetDevice(l.dev->id) != hipSuccess) {
+        fprintf(stderr, "libcuda-crypt: hipSetDevice(%d) failed\n", l.dev->id);
+        l.q->mu.unlock();
+        return Lease{};
+    }
+    return l;
+}
+
+void release_queue(const Lease &l) {
+    if (l.q) l.q->mu.unlock();
+}
+
+const ed25519_tables *device_tables(int hip_device) {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    for (int d = 0; d < g_num_devices; d++)
+        if (g_dev[d].id == hip_device) return g_dev[d].d_tables;
+    return nullptr;
+}
+
+void pool_free_buffers() {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    if (g_num_devices <= 0) return;
+    for (int d = 0; d < g_num_devices; d++) {
+        if (hipSetDevice(g_dev[d].id) != hipSuccess) continue;
+        for (int q = 0; q < kQueuesPerDevice; q++) {
+            QueueCtx &c = g_dev[d].q[q];
+            std::lock_guard<std::mutex> qg(c.mu);
+            (void)hipStreamSynchronize(c.stream);
+            c.d_packets.release();
+            c.d_offsets.release();
+            c.d_out.release();
+            c.d_scratch.release();
+            c.d_hashes.release();
+            c.d_counts.release();
+            c.h_stage.release();
+        }
+    }
+}
+
+}  // namespace spl

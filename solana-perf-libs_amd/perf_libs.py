@@ -1,0 +1,184 @@
+"""ctypes binding of libcuda-crypt.so (the gfx950 build), shaped like the
+reference's consumer `solana_perf::perf_libs`: it binds the same exported C
+symbols with the same argument meaning, plus the device-resident extensions.
+
+Importing this module loads `libcuda-crypt.so` from this directory and raises
+if it is missing: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcuda-crypt.so")
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+
+class GpuElems(ctypes.Structure):
+    """gpu_Elems (src/cuda-ecc-ed25519/ed25519.h:35-38)."""
+    _fields_ = [("elems", _u8p), ("num", ctypes.c_uint32)]
+
+
+# every symbol include/cuda_crypt.h declares
+EXPORTS = (
+    "ed25519_create_seed", "ed25519_create_keypair", "ed25519_sign", "ed25519_sign_many",
+    "ed25519_verify", "ed25519_verify_many", "ed25519_set_verbose", "ed25519_license",
+    "ed25519_init", "cuda_host_register", "cuda_host_unregister",
+    "ed25519_get_checked_scalar", "ed25519_check_packed_ge_small_order",
+    "ed25519_free_gpu_mem", "poh_verify_many_set_verbose", "poh_verify_many",
+    "ed25519_verify_many_device", "ed25519_sign_many_device", "poh_verify_many_device",
+)
+
+
+def _p(a, t=_u8p):
+    return a.ctypes.data_as(t)
+
+
+class PerfLibs:
+    def __init__(self, path=LIB_PATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"{path} is not built; run __graft_entry__.build() (make -C solana-perf-libs_amd)")
+        self.lib = lib = ctypes.CDLL(path)
+        lib.ed25519_create_seed.argtypes = [_u8p]
+        lib.ed25519_create_seed.restype = ctypes.c_int
+        lib.ed25519_create_keypair.argtypes = [_u8p, _u8p, _u8p]
+        lib.ed25519_sign.argtypes = [_u8p, _u8p, ctypes.c_size_t, _u8p, _u8p]
+        lib.ed25519_verify.argtypes = [_u8p, _u8p, ctypes.c_uint32, _u8p]
+        lib.ed25519_verify.restype = ctypes.c_int
+        many = [ctypes.POINTER(GpuElems), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                ctypes.c_uint32, _u32p, _u32p, _u32p, _u32p, _u8p, ctypes.c_uint8]
+        lib.ed25519_verify_many.argtypes = many
+        lib.ed25519_verify_many.restype = None
+        lib.ed25519_sign_many.argtypes = many
+        lib.ed25519_sign_many.restype = None
+        lib.ed25519_set_verbose.argtypes = [ctypes.c_bool]
+        lib.poh_verify_many_set_verbose.argtypes = [ctypes.c_bool]
+        lib.ed25519_license.restype = ctypes.c_char_p
+        lib.ed25519_init.restype = ctypes.c_bool
+        lib.cuda_host_register.argtypes = [_vp, ctypes.c_size_t, ctypes.c_uint]
+        lib.cuda_host_register.restype = ctypes.c_int
+        lib.cuda_host_unregister.argtypes = [_vp]
+        lib.cuda_host_unregister.restype = ctypes.c_int
+        lib.ed25519_get_checked_scalar.argtypes = [_u8p, _u8p]
+        lib.ed25519_get_checked_scalar.restype = ctypes.c_int
+        lib.ed25519_check_packed_ge_small_order.argtypes = [_u8p]
+        lib.ed25519_check_packed_ge_small_order.restype = ctypes.c_int
+        lib.ed25519_free_gpu_mem.restype = None
+        lib.poh_verify_many.argtypes = [_u8p, _u64p, ctypes.c_size_t, ctypes.c_uint8]
+        lib.poh_verify_many.restype = ctypes.c_int
+        dev = [_vp, ctypes.c_uint64, _vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]
+        lib.ed25519_verify_many_device.argtypes = dev
+        lib.ed25519_verify_many_device.restype = ctypes.c_int
+        lib.ed25519_sign_many_device.argtypes = dev
+        lib.ed25519_sign_many_device.restype = ctypes.c_int
+        lib.poh_verify_many_device.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
+        lib.poh_verify_many_device.restype = ctypes.c_int
+
+    # ------------------------------------------------------------ host API
+    def init(self):
+        return bool(self.lib.ed25519_init())
+
+    def license(self):
+        return self.lib.ed25519_license().decode()
+
+    def set_verbose(self, v):
+        self.lib.ed25519_set_verbose(bool(v))
+        self.lib.poh_verify_many_set_verbose(bool(v))
+
+    def create_keypair(self, seed):
+        pk = np.zeros(32, np.uint8)
+        sk = np.zeros(64, np.uint8)
+        s = np.frombuffer(bytes(seed), np.uint8).copy()
+        self.lib.ed25519_create_keypair(_p(pk), _p(sk), _p(s))
+        return pk.tobytes(), sk.tobytes()
+
+    def sign(self, msg, pk, sk):
+        sig = np.zeros(64, np.uint8)
+        m = np.frombuffer(bytes(msg) + b"\0", np.uint8).copy()
+        p = np.frombuffer(bytes(pk), np.uint8).copy()
+        s = np.frombuffer(bytes(sk), np.uint8).copy()
+        self.lib.ed25519_sign(_p(sig), _p(m), len(msg), _p(p), _p(s))
+        return sig.tobytes()
+
+    def verify(self, sig, msg, pk):
+        s = np.frombuffer(bytes(sig), np.uint8).copy()
+        m = np.frombuffer(bytes(msg) + b"\0", np.uint8).copy()
+        p = np.frombuffer(bytes(pk), np.uint8).copy()
+        return self.lib.ed25519_verify(_p(s), _p(m), len(msg), _p(p))
+
+    def get_checked_scalar(self, inp, out_prior=bytes(32)):
+        out = np.frombuffer(bytes(out_prior), np.uint8).copy()
+        i = np.frombuffer(bytes(inp), np.uint8).copy()
+        rc = self.lib.ed25519_get_checked_scalar(_p(out), _p(i))
+        return rc, out.tobytes()
+
+    def check_packed_ge_small_order(self, enc):
+        e = np.frombuffer(bytes(enc), np.uint8).copy()
+        return self.lib.ed25519_check_packed_ge_small_order(_p(e))
+
+    # ------------------------------------------------------------ batch API
+    def verify_many(self, packet_batches, message_size, message_lens, pk_off, sig_off, msg_off,
+                    use_non_default_stream=1):
+        """packet_batches: list of uint8 arrays, each a whole number of packets of
+        message_size bytes (the gpu_Elems list)."""
+        return self._many(self.lib.ed25519_verify_many, packet_batches, message_size,
+                          message_lens, pk_off, sig_off, msg_off, 1, use_non_default_stream)
+
+    def sign_many(self, packet_batches, message_size, message_lens, pk_off, sk_off, msg_off,
+                  use_non_default_stream=1):
+        return self._many(self.lib.ed25519_sign_many, packet_batches, message_size,
+                          message_lens, pk_off, sk_off, msg_off, 64, use_non_default_stream)
+
+    @staticmethod
+    def _many(fn, batches, message_size, lens, t1, t2, t3, out_width, nds):
+        batches = [np.ascontiguousarray(b, dtype=np.uint8) for b in batches]
+        elems = (GpuElems * max(1, len(batches)))()
+        total_packets = 0
+        for k, b in enumerate(batches):
+            assert b.size % message_size == 0
+            elems[k].elems = _p(b)
+            elems[k].num = b.size // message_size
+            total_packets += b.size // message_size
+        tabs = [np.ascontiguousarray(t, dtype=np.uint32) for t in (lens, t1, t2, t3)]
+        n = tabs[0].size
+        out = np.zeros(max(1, n * out_width), np.uint8)
+        fn(elems, len(batches), message_size, total_packets, n,
+           *[_p(t, _u32p) for t in tabs], _p(out), nds)
+        return out[: n * out_width]
+
+    def poh_verify_many(self, hashes, num_hashes, use_non_default_stream=1):
+        """hashes: uint8 [n*32], updated in place. Returns the ABI's int."""
+        nh = np.ascontiguousarray(num_hashes, dtype=np.uint64)
+        assert hashes.dtype == np.uint8 and hashes.flags.c_contiguous and hashes.size == 32 * nh.size
+        return self.lib.poh_verify_many(_p(hashes), _p(nh, _u64p), nh.size, use_non_default_stream)
+
+    # ------------------------------------------------------------ device API
+    def verify_many_device(self, d_packets, packets_size, d_lens, d_pk, d_sig, d_msg, n, d_out,
+                           stream=0):
+        return self.lib.ed25519_verify_many_device(d_packets, packets_size, d_lens, d_pk, d_sig,
+                                                   d_msg, n, d_out, stream)
+
+    def sign_many_device(self, d_packets, packets_size, d_lens, d_pk, d_sk, d_msg, n, d_out,
+                         stream=0):
+        return self.lib.ed25519_sign_many_device(d_packets, packets_size, d_lens, d_pk, d_sk,
+                                                 d_msg, n, d_out, stream)
+
+    def poh_verify_many_device(self, d_hashes, d_counts, n, stream=0):
+        return self.lib.poh_verify_many_device(d_hashes, d_counts, n, stream)
+
+
+_LIB = None
+
+
+def load():
+    """The process-wide PerfLibs instance (raises if the library is not built)."""
+    global _LIB
+    if _LIB is None:
+        _LIB = PerfLibs()
+    return _LIB

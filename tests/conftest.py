@@ -1,0 +1,48 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "solana-perf-libs_amd")
+for p in (ROOT, PKG, os.path.dirname(os.path.abspath(__file__))):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import Oracle
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def reflib():
+    from oracle import RefLib
+    if not RefLib.available():
+        pytest.skip("oracle/_ref/libref.so not built (reference sources absent)")
+    return RefLib()
+
+
+@pytest.fixture(scope="session")
+def lib():
+    import perf_libs
+    return perf_libs.load()
+
+
+@pytest.fixture(scope="session")
+def gpu_lib(lib):
+    """The library with a usable GPU; fails (does not skip) when there is none,
+    so a GPU run can never pass without running the kernels."""
+    assert lib.init(), "ed25519_init() found no usable GPU"
+    return lib
+
+
+def load_json(name):
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", name)) as f:
+        return json.load(f)

@@ -1,0 +1,144 @@
+"""Writes the golden fixtures under tests/golden/ (run from the repo root).
+
+Sources:
+  poh_reference.json   the reference's own PoH test data
+                       (src/poh-verify-test/test_hashes_332_129,
+                       test_num_hashes_arr_332, test_hashes_output_332),
+                       re-encoded from ASCII decimal to hex, data only.
+  rfc8032.json         RFC 8032 section 7.1 TEST 1-3 (public vectors).
+  ref_quirks.json      outputs of the reference's own primitives compiled from
+                       its sources by oracle/Makefile (oracle/_ref/libref.so):
+                       scalar32_reduce via get_checked_scalar with non-zero
+                       prior output buffers, the raw-limb small-order test on
+                       the 8-torsion and non-canonical encodings plus random
+                       encodings, and verify/sign/keypair on seeded cases.
+Requires /root/reference and oracle/_ref/libref.so; the fixtures are committed
+so tests never need either.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "solana-perf-libs_amd"))
+REF = "/root/reference/src/poh-verify-test"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def poh_fixture():
+    def ints(name):
+        with open(os.path.join(REF, name)) as f:
+            return [int(t) for t in f.read().split()]
+    n_bytes = ints("test_num_elems_332")[0]
+    hashes = bytes(ints("test_hashes_332_129")[:n_bytes])
+    counts = ints("test_num_hashes_arr_332")[: n_bytes // 32]
+    outputs = bytes(ints("test_hashes_output_332")[:n_bytes])
+    n = n_bytes // 32
+    return {
+        "source": "src/poh-verify-test (test_hashes_332_129, test_num_hashes_arr_332, "
+                  "test_hashes_output_332; test_num_elems_332 = bytes)",
+        "num_chains": n,
+        "inputs": [hashes[32 * i:32 * i + 32].hex() for i in range(n)],
+        "num_hashes": counts,
+        "outputs": [outputs[32 * i:32 * i + 32].hex() for i in range(n)],
+    }
+
+
+RFC8032 = [
+    {"secret": "9d61b19deffd5a60ba844af492ec2cc44449c5697b326919703bac031cae7f60",
+     "public": "d75a980182b10ab7d54bfed3c964073a0ee172f3daa62325af021a68f707511a",
+     "message": "",
+     "signature": "e5564300c360ac729086e2cc806e828a84877f1eb8e5d974d873e06522490155"
+                  "5fb8821590a33bacc61e39701cf9b46bd25bf5f0595bbe24655141438e7a100b"},
+    {"secret": "4ccd089b28ff96da9db6c346ec114e0f5b8a319f35aba624da8cf6ed4fb8a6fb",
+     "public": "3d4017c3e843895a92b70aa74d1b7ebc9c982ccf2ec4968cc0cd55f12af4660c",
+     "message": "72",
+     "signature": "92a009a9f0d4cab8720e820b5f642540a2b27b5416503f8fb3762223ebdb69da"
+                  "085ac1e43e15996e458f3613d0f11d8c387b2eaeb4302aeeb00d291612bb0c00"},
+    {"secret": "c5aa8df43f9f837bedb7442f31dcb7b166d38535076f094b85ce3a2e0b4458f7",
+     "public": "fc51cd8e6218a1a38da47ed00230f0580816ed13ba3303ac5deb911548908025",
+     "message": "af82",
+     "signature": "6291d657deec24024827e69c3abe01a30ce548a284743a445e3680d7db5ac3ac"
+                  "18ff9b538d16f290ae67f760984dc6594a7c15e9716ed28dc027beceea1ec40a"},
+]
+
+
+def ref_quirks():
+    from oracle import RefLib
+    import workloads as W
+    ref = RefLib()
+    rng = np.random.default_rng(20240601)
+    L = W.L_ORDER
+
+    # get_checked_scalar: every branch, with zero and random prior buffers
+    gcs = []
+    for k in range(600):
+        inp = bytearray(rng.integers(0, 256, 32, dtype=np.uint8).tobytes())
+        sel = k % 6
+        if sel == 0:
+            inp[31] &= 0x0F                                  # reduced nibble
+        elif sel == 1:
+            inp[31] |= 0x80                                  # top bit set
+        elif sel in (2, 3):
+            inp[31] = 0x10 + int(rng.integers(0, 0x70))      # quirk path
+        elif sel == 4:
+            v = (int(rng.integers(0, 2**62)) * 2**190 + L) % (2**255)
+            inp = bytearray(v.to_bytes(32, "little"))        # near multiples of L
+        prior = bytes(32) if k % 2 == 0 else rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        if k % 10 == 9:                                      # prior = scalar < L
+            prior = (int.from_bytes(rng.integers(0, 256, 32, dtype=np.uint8).tobytes(),
+                                    "little") % L).to_bytes(32, "little")
+        rc, out = ref.get_checked_scalar(bytes(inp), prior, 0)
+        gcs.append({"in": bytes(inp).hex(), "prior": prior.hex(), "rc": rc, "out": out.hex()})
+
+    # raw-limb small-order test on the special encodings and random ones
+    special = W.TORSION_CANONICAL + W.TORSION_NONCANONICAL
+    so = [{"enc": e.hex(), "raw": ref.small_order_raw(e), "check": ref.check_small_order(e)}
+          for e in special]
+    for k in range(300):
+        e = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        so.append({"enc": e.hex(), "raw": ref.small_order_raw(e),
+                   "check": ref.check_small_order(e)})
+
+    # seeded keypair / sign / verify cases incl. adversarial encodings
+    cases = []
+    for k in range(64):
+        seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        pk, sk = ref.create_keypair(seed)
+        msg = rng.integers(0, 256, int(rng.integers(0, 300)), dtype=np.uint8).tobytes()
+        sig = bytearray(ref.sign(msg, pk, sk))
+        kind = k % 8
+        if kind == 1:
+            sig[int(rng.integers(0, 64))] ^= 1 << int(rng.integers(0, 8))
+        elif kind == 2:
+            s = (int.from_bytes(sig[32:], "little") + L).to_bytes(32, "little")
+            sig[32:] = s
+        elif kind == 3:
+            pk = special[int(rng.integers(0, len(special)))]
+        elif kind == 4:
+            sig[:32] = special[int(rng.integers(0, len(special)))]
+        elif kind == 5:
+            sig[63] |= 0x10                                  # S in [2^252, 2^253)
+        cases.append({"seed": seed.hex(), "pk": bytes(pk).hex(), "sk": sk.hex(),
+                      "msg": msg.hex(), "sig": bytes(sig).hex(), "kind": kind,
+                      "verify": ref.verify(bytes(sig), msg, pk)})
+    return {"source": "oracle/_ref/libref.so built from src/cuda-ecc-ed25519 "
+                      "(fe.cu ge.cu sc.cu sha512.cu keypair.cu) and src/cuda-sha256",
+            "get_checked_scalar": gcs, "small_order": so, "cases": cases}
+
+
+def main():
+    with open(os.path.join(OUT, "poh_reference.json"), "w") as f:
+        json.dump(poh_fixture(), f, indent=0)
+    with open(os.path.join(OUT, "rfc8032.json"), "w") as f:
+        json.dump(RFC8032, f, indent=1)
+    with open(os.path.join(OUT, "ref_quirks.json"), "w") as f:
+        json.dump(ref_quirks(), f, indent=0)
+    print("fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()

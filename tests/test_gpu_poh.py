@@ -1,0 +1,82 @@
+"""GPU parity of poh_verify_many (and its device-resident form) against the
+oracle and the reference's own PoH fixture; bit-exact end hashes."""
+import numpy as np
+import pytest
+
+from conftest import load_json
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_reference_fixture(gpu_lib):
+    fx = load_json("poh_reference.json")
+    h = np.frombuffer(bytes.fromhex("".join(fx["inputs"])), np.uint8).copy()
+    counts = np.array(fx["num_hashes"], np.uint64)
+    for nds in (0, 1):
+        x = h.copy()
+        assert gpu_lib.poh_verify_many(x, counts, nds) == 0
+        assert x.tobytes().hex() == "".join(fx["outputs"])
+
+
+def test_empty_and_zero_counts(gpu_lib, oracle):
+    x = np.zeros(0, np.uint8)
+    assert gpu_lib.poh_verify_many(x, np.zeros(0, np.uint64)) == 0
+    h, _ = W.poh_chains(7, 0, seed=9)
+    counts = np.array([0, 1, 0, 2, 5, 0, 33], np.uint64)
+    want = h.copy()
+    oracle.poh_verify_many(want, counts)
+    got = h.copy()
+    assert gpu_lib.poh_verify_many(got, counts) == 0
+    assert np.array_equal(got, want)
+    assert np.array_equal(got[:32], h[:32])          # count 0: unchanged
+
+
+def test_ragged_counts_vs_oracle(gpu_lib, oracle):
+    rng = np.random.default_rng(4)
+    n = 1000
+    h, _ = W.poh_chains(n, 0, seed=10)
+    counts = rng.integers(0, 700, n).astype(np.uint64)
+    want = h.copy()
+    oracle.poh_verify_many(want, counts, threads=8)
+    got = h.copy()
+    assert gpu_lib.poh_verify_many(got, counts) == 0
+    assert np.array_equal(got, want)
+
+
+def test_c2_shape_vs_oracle(gpu_lib, oracle):
+    """4096 chains (the plain poh_verify_many workload) at a reduced count, and
+    a 64-chain slice at the full 12,500 hashes; 5% of expected hashes have a
+    bit flipped and must be reported as mismatches."""
+    h, counts = W.poh_chains(4096, 1000, seed=2)
+    want = h.copy()
+    oracle.poh_verify_many(want, counts, threads=8)
+    got = h.copy()
+    assert gpu_lib.poh_verify_many(got, counts) == 0
+    assert np.array_equal(got, want)
+
+    rng = np.random.default_rng(12)
+    h2, c2 = W.poh_chains(64, 12500, seed=3)
+    expected = h2.copy()
+    oracle.poh_verify_many(expected, c2, threads=8)
+    flip = rng.random(64) < 0.05
+    flip[0] = True
+    for i in np.nonzero(flip)[0]:
+        expected[32 * i + int(rng.integers(0, 32))] ^= np.uint8(1 << int(rng.integers(0, 8)))
+    got = h2.copy()
+    assert gpu_lib.poh_verify_many(got, c2) == 0
+    mask = (got.reshape(-1, 32) == expected.reshape(-1, 32)).all(axis=1)
+    assert np.array_equal(mask, ~flip)
+
+
+def test_device_resident_form(gpu_lib, oracle):
+    import torch
+    h, counts = W.poh_chains(3000, 37, seed=8)
+    want = h.copy()
+    oracle.poh_verify_many(want, counts, threads=8)
+    dh = torch.from_numpy(h.copy()).cuda()
+    dc = torch.from_numpy(counts.view(np.int64)).cuda()
+    s = torch.cuda.current_stream()
+    assert gpu_lib.poh_verify_many_device(dh.data_ptr(), dc.data_ptr(), 3000, s.cuda_stream) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(dh.cpu().numpy(), want)

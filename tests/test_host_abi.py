@@ -1,0 +1,126 @@
+"""CPU tests of libcuda-crypt.so: it loads, exports every symbol the header
+declares, and its host-side exports (which run the same field/group/scalar
+code as the kernels, compiled for the host) agree with the oracle and the
+committed fixtures.  No GPU compute is called here."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+
+from conftest import ROOT, load_json
+import ed25519_math as E
+import perf_libs
+import workloads as W
+
+
+def test_exports_match_header(lib):
+    hdr = open(os.path.join(ROOT, "include", "cuda_crypt.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    declared = set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9 ]+\*?\s*\b([a-z_0-9]+)\(", hdr, re.M))
+    declared -= {"if", "return"}
+    assert declared == set(perf_libs.EXPORTS)
+    out = subprocess.run(["nm", "-D", "--defined-only", perf_libs.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = declared - exported
+    assert not missing, missing
+
+
+def test_license_string(lib):
+    assert "Apache License, Version 2.0" in lib.license()
+
+
+def test_rfc8032(lib):
+    for v in load_json("rfc8032.json"):
+        pk, sk = lib.create_keypair(bytes.fromhex(v["secret"]))
+        assert pk.hex() == v["public"]
+        msg = bytes.fromhex(v["message"])
+        assert lib.sign(msg, pk, sk).hex() == v["signature"]
+        assert lib.verify(bytes.fromhex(v["signature"]), msg, pk) == 1
+
+
+def test_host_exports_vs_reference_fixture(lib):
+    fx = load_json("ref_quirks.json")
+    for c in fx["get_checked_scalar"]:
+        rc, out = lib.get_checked_scalar(bytes.fromhex(c["in"]), bytes.fromhex(c["prior"]))
+        assert (rc, out.hex()) == (c["rc"], c["out"]), c
+    for c in fx["small_order"]:
+        assert lib.check_packed_ge_small_order(bytes.fromhex(c["enc"])) == c["check"]
+    for c in fx["cases"]:
+        pk, sk = lib.create_keypair(bytes.fromhex(c["seed"]))
+        assert sk.hex() == c["sk"]
+        sig, msg, cpk = bytes.fromhex(c["sig"]), bytes.fromhex(c["msg"]), bytes.fromhex(c["pk"])
+        assert lib.verify(sig, msg, cpk) == c["verify"], c["kind"]
+        if c["kind"] in (0, 6, 7):
+            assert lib.sign(msg, pk, sk) == sig
+
+
+def test_host_exports_vs_oracle_random(lib, oracle):
+    rng = np.random.default_rng(17)
+    for k in range(60):
+        seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        pk, sk = lib.create_keypair(seed)
+        assert (pk, sk) == oracle.create_keypair(seed)
+        msg = rng.integers(0, 256, int(rng.integers(0, 1300)), dtype=np.uint8).tobytes()
+        sig = lib.sign(msg, pk, sk)
+        assert sig == oracle.sign(msg, pk, sk)
+        assert lib.verify(sig, msg, pk) == 1
+        bad = bytearray(sig)
+        bad[int(rng.integers(0, 64))] ^= 1 << int(rng.integers(0, 8))
+        assert lib.verify(bytes(bad), msg, pk) == oracle.verify(bytes(bad), msg, pk)
+        enc = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        assert lib.check_packed_ge_small_order(enc) == oracle.check_packed_ge_small_order(enc)
+        inp = bytearray(rng.integers(0, 256, 32, dtype=np.uint8).tobytes())
+        inp[31] = [0x05, 0x17, 0x42, 0x7f, 0x80, 0xff][k % 6]
+        prior = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        assert lib.get_checked_scalar(bytes(inp), prior) == \
+            oracle.get_checked_scalar(bytes(inp), prior)
+
+
+def test_verify_edge_encodings(lib):
+    """Non-canonical R never verifies; small-order keys and R rejected;
+    S >= 2^252 rejected (host build of the verify core)."""
+    seed = bytes(range(32))
+    pk, sk = lib.create_keypair(seed)
+    msg = b"edge cases"
+    sig = lib.sign(msg, pk, sk)
+    assert lib.verify(sig, msg, pk) == 1
+    s_plus_l = (int.from_bytes(sig[32:], "little") + E.L).to_bytes(32, "little")
+    assert lib.verify(sig[:32] + s_plus_l, msg, pk) == 0
+    for enc in W.SMALL_ORDER_KEYS:
+        assert lib.verify(sig, msg, enc) == E.verify_rule(sig, msg, enc) == 0
+        assert lib.verify(enc + sig[32:], msg, pk) == E.verify_rule(enc + sig[32:], msg, pk)
+
+
+def _run_without_gpu(code):
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1",
+               CUDA_VISIBLE_DEVICES="-1")
+    return subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                          env=env, cwd=ROOT, timeout=120)
+
+
+def test_no_gpu_behaviour():
+    """Without a device: init is false, poh_verify_many returns 1 and leaves the
+    hashes alone, verify_many rejects everything instead of crashing
+    (the reference dereferences NULL there, verify.cu:265-267)."""
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, 'solana-perf-libs_amd')
+import perf_libs
+L = perf_libs.load()
+assert L.init() is False
+h = np.arange(64, dtype=np.uint8); h0 = h.copy()
+assert L.poh_verify_many(h, np.array([3, 4], np.uint64)) == 1
+assert np.array_equal(h, h0)
+pk, sk = L.create_keypair(bytes(32))
+pkt = np.zeros(256, np.uint8)
+pkt[:64] = np.frombuffer(L.sign(b'x' * 100, pk, sk), np.uint8)
+pkt[64:96] = np.frombuffer(pk, np.uint8); pkt[96:196] = ord('x')
+out = L.verify_many([pkt], 256, [100], [64], [0], [96])
+assert out.tolist() == [0]
+print('ok')
+"""
+    r = _run_without_gpu(code)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
