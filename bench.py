@@ -98,42 +98,40 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    import hip_rt
     import perf_libs
     import workloads as W
     lib = perf_libs.load()
     assert lib.init(), "no usable GPU"
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
+    stream = hip_rt.Stream()
 
     log("bench: generating validator-shaped batch")
     b = W.verify_validator(lib.sign, lib.create_keypair, seed=3)
-    tens = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    d_packets = tens(b.packets)
-    d_tabs = [tens(x.view(np.int32)) for x in (b.lens, b.pk_off, b.sig_off, b.msg_off)]
-    d_out = torch.zeros(b.n, dtype=torch.uint8, device=dev)
+    d_packets = hip_rt.DeviceBuffer.from_array(b.packets)
+    d_tabs = [hip_rt.DeviceBuffer.from_array(x) for x in (b.lens, b.pk_off, b.sig_off, b.msg_off)]
+    d_out = hip_rt.DeviceBuffer(b.n)
 
     def step():
-        rc = lib.verify_many_device(d_packets.data_ptr(), b.packets.size,
-                                    *[x.data_ptr() for x in d_tabs], b.n, d_out.data_ptr(), sptr)
+        rc = lib.verify_many_device(d_packets.ptr, b.packets.size, *[x.ptr for x in d_tabs], b.n,
+                                    d_out.ptr, stream.handle)
         assert rc == 0
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    stream.synchronize()
     log(f"bench: timing {args.steps} steps of {b.n} signatures")
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    ev0, ev1 = hip_rt.Event(), hip_rt.Event()
+    hip_rt.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
         step()
     ev1.record(stream)
-    torch.cuda.synchronize()
+    ev1.synchronize()
+    hip_rt.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = ev0.elapsed_time(ev1) / args.steps
-    mask = d_out.cpu().numpy()
+    kern_ms = ev0.elapsed_ms(ev1) / args.steps
+    mask = d_out.download()
     ms_per_step = wall * 1e3 / args.steps
     value = b.n / (ms_per_step / 1e3)
     ops = verify_ops(b.lens)
@@ -142,7 +140,7 @@ def main():
         "metric": "sig verifies/s + PoH SHA-256 hashes/s on 1 MI355X; p99 request latency",
         "value": value, "unit": "sig-verifies/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32 (u32 limbs, exact)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32/uint32 (exact integer)",
         "data": "synthetic, seeded (numpy PCG64); keys/signatures made by the library's host "
                 "ed25519_create_keypair/ed25519_sign",
         "config": {"workload": "validator-shaped batched verify: 128 batches x U[1,256] packets, "
@@ -161,16 +159,16 @@ def main():
         poh = {}
         for name, n, reps in (("c2_4096x12500", 4096, 12500), ("sat_262144x500", 262144, 500)):
             h, counts = W.poh_chains(n, reps, seed=2)
-            dh = tens(h)
-            dc = tens(counts.view(np.int64))
-            lib.poh_verify_many_device(dh.data_ptr(), dc.data_ptr(), n, sptr)
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            dh = hip_rt.DeviceBuffer.from_array(h)
+            dc = hip_rt.DeviceBuffer.from_array(counts)
+            lib.poh_verify_many_device(dh.ptr, dc.ptr, n, stream.handle)
+            stream.synchronize()
+            e0, e1 = hip_rt.Event(), hip_rt.Event()
             e0.record(stream)
-            lib.poh_verify_many_device(dh.data_ptr(), dc.data_ptr(), n, sptr)
+            lib.poh_verify_many_device(dh.ptr, dc.ptr, n, stream.handle)
             e1.record(stream)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1)
             hps = n * reps / (ms / 1e3)
             poh[name] = {"hashes_per_s": hps, "ms": ms,
                          "roofline_frac": hps * W_SHA256 / VALU_PEAK}

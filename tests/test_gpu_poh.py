@@ -70,13 +70,13 @@ def test_c2_shape_vs_oracle(gpu_lib, oracle):
 
 
 def test_device_resident_form(gpu_lib, oracle):
-    import torch
+    import hip_rt
     h, counts = W.poh_chains(3000, 37, seed=8)
     want = h.copy()
     oracle.poh_verify_many(want, counts, threads=8)
-    dh = torch.from_numpy(h.copy()).cuda()
-    dc = torch.from_numpy(counts.view(np.int64)).cuda()
-    s = torch.cuda.current_stream()
-    assert gpu_lib.poh_verify_many_device(dh.data_ptr(), dc.data_ptr(), 3000, s.cuda_stream) == 0
-    torch.cuda.synchronize()
-    assert np.array_equal(dh.cpu().numpy(), want)
+    dh = hip_rt.DeviceBuffer.from_array(h)
+    dc = hip_rt.DeviceBuffer.from_array(counts)
+    s = hip_rt.Stream()
+    assert gpu_lib.poh_verify_many_device(dh.ptr, dc.ptr, 3000, s.handle) == 0
+    s.synchronize()
+    assert np.array_equal(dh.download(), want)

@@ -125,17 +125,16 @@ def test_empty_and_bad_offsets(gpu_lib, oracle):
 
 
 def test_device_resident_form(gpu_lib, oracle):
-    import torch
+    import hip_rt
     b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=11, num_batches=6,
                            num_keys=64)
     want = _oracle_mask(oracle, b)
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    dp = t(b.packets)
-    tabs = [t(x.view(np.int32)) for x in (b.lens, b.pk_off, b.sig_off, b.msg_off)]
-    dout = torch.zeros(b.n, dtype=torch.uint8, device="cuda")
-    s = torch.cuda.current_stream().cuda_stream
-    rc = gpu_lib.verify_many_device(dp.data_ptr(), b.packets.size,
-                                    *[x.data_ptr() for x in tabs], b.n, dout.data_ptr(), s)
+    dp = hip_rt.DeviceBuffer.from_array(b.packets)
+    tabs = [hip_rt.DeviceBuffer.from_array(x) for x in (b.lens, b.pk_off, b.sig_off, b.msg_off)]
+    dout = hip_rt.DeviceBuffer(b.n)
+    s = hip_rt.Stream()
+    rc = gpu_lib.verify_many_device(dp.ptr, b.packets.size, *[x.ptr for x in tabs], b.n,
+                                    dout.ptr, s.handle)
     assert rc == 0
-    torch.cuda.synchronize()
-    assert np.array_equal(dout.cpu().numpy(), want)
+    s.synchronize()
+    assert np.array_equal(dout.download(), want)
