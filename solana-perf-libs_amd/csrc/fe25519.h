@@ -1,17 +1,30 @@
-// GF(2^255 - 19) arithmetic for gfx950, radix 2^32.
+// GF(2^255 - 19) arithmetic for gfx950.
 //
-// The reference keeps field elements as 10 signed limbs in radix 2^25.5
-// (src/cuda-ecc-ed25519/fe.h:7-16, fe.cu) so that CUDA's 32x32->64 IMAD can
-// accumulate 100 limb products per multiply.  On CDNA4 the natural multiplier
-// is V_MAD_U64_U32 (32x32 + 64 -> 64, one instruction), so the radix here is
-// 2^32: 8 unsigned words, 64 limb products per multiply and 36 per square.
+// Radix choice (measured on MI355X, gfx950):
+//   V_MAD_U64_U32 / V_MAD_I64_I32 (32x32 + 64 -> 64) issue at ~0.8-0.9 of the
+//   int32 VALU peak, i.e. a full-rate multiply-accumulate.  What decides the
+//   cost of a field multiply is therefore the bookkeeping around it:
+//     - radix 2^32 (8 words): 64 products, but every product needs a carry
+//       out of the 64-bit accumulator (V_MAD_U64_U32's SGPR carry + an ADDC,
+//       with SGPR hazard NOPs between them) and every add/sub needs a carry
+//       chain plus a fold of 2^256 = 38;
+//     - radix 2^25.5 (10 signed limbs of alternately 26 and 25 bits): 100
+//       products, none of which can overflow a signed 64-bit column sum, the
+//       wrap-around factor 19 is folded into a 32-bit multiplier operand
+//       (19 * limb < 2^31), squares need only 55 products, and add/sub are 10
+//       independent 32-bit adds.
+//   Both multiplies measured ~175 lane-cycles; the 2^25.5 form wins on
+//   squares (55 vs 36 products + carries + doubling) and on add/sub, which
+//   dominate the point formulas, so it is used here.
 //
-// Representation: w[0..7] little-endian words of an integer v < 2^256 with
-// v == x (mod p).  Values are kept only "weakly" reduced (mod 2^256 - 38);
-// fe_canon() produces the unique representative in [0, p).
+// Element: v[i] has weight 2^ceil(25.5 i).  "Tight" elements (outputs of mul,
+// sq, carry) have |v_even| <= ~2^25 and |v_odd| <= ~2^24; a sum or difference
+// of at most three tight elements is a valid input to mul/sq (the column sums
+// then stay below 2^62).  fe_to_words / fe_canon give the unique
+// representative in [0, p).
 //
-// Every routine is __host__ __device__ so the exported host helpers
-// (ed25519_verify, ed25519_check_packed_ge_small_order, ...) run the same code.
+// Every routine is __host__ __device__ so the exported host helpers run the
+// same arithmetic as the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -21,140 +34,147 @@
 namespace spl {
 
 struct fe {
-    uint32_t w[8];
+    int32_t v[10];
 };
 
 SPL_HD uint32_t lo32(uint64_t x) { return (uint32_t)x; }
 SPL_HD uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
 
-SPL_HD void fe_set(fe &r, uint32_t v) {
-    r.w[0] = v;
+// limb i holds 26 bits when i is even, 25 bits when odd
+#define SPL_LIMB_BITS(i) (((i) & 1) ? 25 : 26)
+
+SPL_HD void fe_set(fe &r, int32_t small) {
+    r.v[0] = small;
 #pragma unroll
-    for (int i = 1; i < 8; i++) r.w[i] = 0;
+    for (int i = 1; i < 10; i++) r.v[i] = 0;
 }
 
-// r = a + b (mod 2^256 - 38)
 SPL_HD void fe_add(fe &r, const fe &a, const fe &b) {
-    uint64_t c = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        c = (uint64_t)a.w[i] + b.w[i] + hi32(c);
-        r.w[i] = lo32(c);
-    }
-    // 2^256 == 38: fold the carry, then fold the (rare) second carry, which
-    // can only occur when the result is below 38 and so cannot propagate.
-    c = (uint64_t)r.w[0] + 38u * hi32(c);
-    r.w[0] = lo32(c);
-#pragma unroll
-    for (int i = 1; i < 8; i++) {
-        c = (uint64_t)r.w[i] + hi32(c);
-        r.w[i] = lo32(c);
-    }
-    r.w[0] += 38u * hi32(c);
+    for (int i = 0; i < 10; i++) r.v[i] = a.v[i] + b.v[i];
 }
 
-// r = a - b (mod 2^256 - 38)
 SPL_HD void fe_sub(fe &r, const fe &a, const fe &b) {
-    int64_t c = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        c = (int64_t)a.w[i] - b.w[i] + (c >> 32);
-        r.w[i] = (uint32_t)c;
-    }
-    // a borrow means 2^256 was added: take 38 back off (twice at most; the
-    // second time only touches w[0], which is then >= 2^32 - 38).
-    uint32_t bw = (uint32_t)(-(c >> 32));
-    c = (int64_t)r.w[0] - 38 * (int64_t)bw;
-    r.w[0] = (uint32_t)c;
-#pragma unroll
-    for (int i = 1; i < 8; i++) {
-        c = (int64_t)r.w[i] + (c >> 32);
-        r.w[i] = (uint32_t)c;
-    }
-    bw = (uint32_t)(-(c >> 32));
-    r.w[0] -= 38u * bw;
+    for (int i = 0; i < 10; i++) r.v[i] = a.v[i] - b.v[i];
 }
 
 SPL_HD void fe_neg(fe &r, const fe &a) {
-    fe z;
-    fe_set(z, 0);
-    fe_sub(r, z, a);
+#pragma unroll
+    for (int i = 0; i < 10; i++) r.v[i] = -a.v[i];
 }
 
-// 512-bit t[0..15] -> r (mod 2^256 - 38): r = t_lo + 38 t_hi.
-SPL_HD void fe_fold512(fe &r, const uint32_t t[16]) {
-    uint64_t c = 0;
+SPL_HD void fe_cmov(fe &r, const fe &a, uint32_t flag) {
+    int32_t m = -(int32_t)flag;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        c = (uint64_t)t[8 + i] * 38u + t[i] + hi32(c);
-        r.w[i] = lo32(c);
-    }
-    // hi32(c) < 39: fold once more with a full carry chain, then the last
-    // carry (only when the value is tiny) into w[0].
-    c = (uint64_t)hi32(c) * 38u + r.w[0];
-    r.w[0] = lo32(c);
-#pragma unroll
-    for (int i = 1; i < 8; i++) {
-        c = (uint64_t)r.w[i] + hi32(c);
-        r.w[i] = lo32(c);
-    }
-    r.w[0] += 38u * hi32(c);
+    for (int i = 0; i < 10; i++) r.v[i] ^= (r.v[i] ^ a.v[i]) & m;
 }
 
-// r = a * b
-SPL_HD void fe_mul(fe &r, const fe &a, const fe &b) {
-    uint32_t t[16];
-    uint64_t c = 0;
+// Rounding carry of limb i into limb i+1 (limb 9 wraps into limb 0 times 19).
+template <typename T>
+SPL_HD void carry_round(T c[10], int i) {
+    const int b = SPL_LIMB_BITS(i);
+    T cy = (c[i] + ((T)1 << (b - 1))) >> b;
+    c[i] -= cy * ((T)1 << b);
+    if (i == 9) c[0] += 19 * cy;
+    else c[i + 1] += cy;
+}
+
+// 64-bit column sums -> tight limbs.  Two interleaved carry chains
+// (0..4 and 4..9) halve the dependency depth.
+SPL_HD void fe_reduce_cols(fe &h, int64_t c[10]) {
+    carry_round(c, 0); carry_round(c, 4);
+    carry_round(c, 1); carry_round(c, 5);
+    carry_round(c, 2); carry_round(c, 6);
+    carry_round(c, 3); carry_round(c, 7);
+    carry_round(c, 4); carry_round(c, 8);
+    carry_round(c, 9);
+    carry_round(c, 0);
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        c = (uint64_t)a.w[0] * b.w[j] + hi32(c);
-        t[j] = lo32(c);
+    for (int i = 0; i < 10; i++) h.v[i] = (int32_t)c[i];
+}
+
+// Loose 32-bit limbs -> tight limbs.
+SPL_HD void fe_carry(fe &h) {
+    int32_t *c = h.v;
+    carry_round(c, 0); carry_round(c, 4);
+    carry_round(c, 1); carry_round(c, 5);
+    carry_round(c, 2); carry_round(c, 6);
+    carry_round(c, 3); carry_round(c, 7);
+    carry_round(c, 4); carry_round(c, 8);
+    carry_round(c, 9);
+    carry_round(c, 0);
+}
+
+// h = f * g.  Column k collects f_i g_j for i + j = k (weight 1, or 2 when i
+// and j are both odd: their weights add up half a bit above limb k's) and, with
+// the factor 19 for 2^255 = 19, the products with i + j = k + 10.
+SPL_HD void fe_mul(fe &h, const fe &f, const fe &g) {
+    int32_t g19[10], f2[10];
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+        g19[i] = 19 * g.v[i];
+        f2[i] = (i & 1) ? 2 * f.v[i] : f.v[i];
     }
-    t[8] = hi32(c);
+    int64_t c[10];
 #pragma unroll
-    for (int i = 1; i < 8; i++) {
-        c = 0;
+    for (int k = 0; k < 10; k++) {
+        int64_t s = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            c = (uint64_t)a.w[i] * b.w[j] + t[i + j] + hi32(c);
-            t[i + j] = lo32(c);
+        for (int i = 0; i < 10; i++) {
+            int j = k - i;
+            const bool wrap = j < 0;
+            if (wrap) j += 10;
+            const int32_t a = ((i & 1) && (j & 1)) ? f2[i] : f.v[i];
+            const int32_t b = wrap ? g19[j] : g.v[j];
+            s += (int64_t)a * b;
         }
-        t[i + 8] = hi32(c);
+        c[k] = s;
     }
-    fe_fold512(r, t);
+    fe_reduce_cols(h, c);
 }
 
-// r = a^2: 28 cross products once, doubled by a shift, plus 8 squares.
-SPL_HD void fe_sq(fe &r, const fe &a) {
-    uint32_t t[16];
+// Column sums of f^2: each unordered pair once, the factor 2 of the cross
+// terms on the left operand and the odd-odd factor 2 / wrap factor 19 on the
+// right one (38 f_j only for odd, 25-bit, limbs so it fits in 31 bits).
+SPL_HD void fe_sq_cols(int64_t c[10], const fe &f) {
+    int32_t f2[10], f19[10], f38[10];
 #pragma unroll
-    for (int i = 0; i < 16; i++) t[i] = 0;
+    for (int i = 0; i < 10; i++) {
+        f2[i] = 2 * f.v[i];
+        f19[i] = 19 * f.v[i];
+        f38[i] = 38 * f.v[i];
+    }
 #pragma unroll
-    for (int i = 0; i < 7; i++) {
-        uint64_t c = 0;
+    for (int k = 0; k < 10; k++) c[k] = 0;
 #pragma unroll
-        for (int j = i + 1; j < 8; j++) {
-            c = (uint64_t)a.w[i] * a.w[j] + t[i + j] + hi32(c);
-            t[i + j] = lo32(c);
+    for (int i = 0; i < 10; i++) {
+#pragma unroll
+        for (int j = i; j < 10; j++) {
+            const bool wrap = i + j >= 10;
+            const bool oo = (i & 1) && (j & 1);
+            const int32_t a = (i == j) ? f.v[i] : f2[i];
+            int32_t b;
+            if (wrap) b = oo ? f38[j] : f19[j];
+            else b = oo ? f2[j] : f.v[j];
+            c[(i + j) % 10] += (int64_t)a * b;
         }
-        t[i + 8] = hi32(c);
     }
-    // double: t <<= 1 (t < 2^511 before doubling)
-    t[15] = (t[15] << 1) | (t[14] >> 31);
+}
+
+SPL_HD void fe_sq(fe &h, const fe &f) {
+    int64_t c[10];
+    fe_sq_cols(c, f);
+    fe_reduce_cols(h, c);
+}
+
+// h = 2 f^2
+SPL_HD void fe_sq2(fe &h, const fe &f) {
+    int64_t c[10];
+    fe_sq_cols(c, f);
 #pragma unroll
-    for (int i = 14; i > 0; i--) t[i] = (t[i] << 1) | (t[i - 1] >> 31);
-    t[0] <<= 1;
-    // add the diagonal a_i^2 at word 2i
-    uint64_t c = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        uint64_t sq = (uint64_t)a.w[i] * a.w[i];
-        c = (uint64_t)t[2 * i] + lo32(sq) + hi32(c);
-        t[2 * i] = lo32(c);
-        c = (uint64_t)t[2 * i + 1] + hi32(sq) + hi32(c);
-        t[2 * i + 1] = lo32(c);
-    }
-    fe_fold512(r, t);
+    for (int k = 0; k < 10; k++) c[k] += c[k];
+    fe_reduce_cols(h, c);
 }
 
 SPL_HD void fe_sqn(fe &r, const fe &a, int n) {
@@ -162,86 +182,91 @@ SPL_HD void fe_sqn(fe &r, const fe &a, int n) {
     for (int i = 1; i < n; i++) fe_sq(r, r);
 }
 
-// r = a * k for a small constant k < 2^26
-SPL_HD void fe_mul_small(fe &r, const fe &a, uint32_t k) {
-    uint64_t c = 0;
+// Unique representative in [0, p), as limbs in [0, 2^26) / [0, 2^25).
+SPL_HD void fe_canon(fe &r, const fe &a) {
+    int32_t h[10];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        c = (uint64_t)a.w[i] * k + hi32(c);
-        r.w[i] = lo32(c);
-    }
-    c = (uint64_t)hi32(c) * 38u + r.w[0];
-    r.w[0] = lo32(c);
+    for (int i = 0; i < 10; i++) h[i] = a.v[i];
+    // tight, then + 2p so every limb is positive
+    fe t;
 #pragma unroll
-    for (int i = 1; i < 8; i++) {
-        c = (uint64_t)r.w[i] + hi32(c);
-        r.w[i] = lo32(c);
+    for (int i = 0; i < 10; i++) t.v[i] = h[i];
+    fe_carry(t);
+#pragma unroll
+    for (int i = 0; i < 10; i++)
+        h[i] = t.v[i] + ((i & 1) ? 2 * 0x1ffffff : (i == 0 ? 2 * 0x3ffffed : 2 * 0x3ffffff));
+    // two floor-carry passes: limbs in range, value < 2^255
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int b = SPL_LIMB_BITS(i);
+            int32_t cy = h[i] >> b;
+            h[i] &= (1 << b) - 1;
+            h[i + 1] += cy;
+        }
+        int32_t cy = h[9] >> 25;
+        h[9] &= (1 << 25) - 1;
+        h[0] += 19 * cy;
     }
-    r.w[0] += 38u * hi32(c);
+    // subtract p once if value >= p, i.e. if value + 19 >= 2^255
+    int32_t s[10];
+    int32_t cy = 19;
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+        const int b = SPL_LIMB_BITS(i);
+        int32_t x = h[i] + cy;
+        cy = x >> b;
+        s[i] = x & ((1 << b) - 1);
+    }
+    const int32_t m = -cy;   // cy is 1 exactly when value >= p
+#pragma unroll
+    for (int i = 0; i < 10; i++) r.v[i] = (h[i] & ~m) | (s[i] & m);
 }
 
-// Unique representative in [0, p).
-SPL_HD void fe_canon(fe &r, const fe &a) {
-    // fold bit 255: v = (v mod 2^255) + 19 * bit255  (< 2^255 + 19)
-    uint32_t top = a.w[7] >> 31;
-    uint64_t c = (uint64_t)a.w[0] + 19u * top;
-    r.w[0] = lo32(c);
-#pragma unroll
-    for (int i = 1; i < 7; i++) {
-        c = (uint64_t)a.w[i] + hi32(c);
-        r.w[i] = lo32(c);
-    }
-    c = (uint64_t)(a.w[7] & 0x7fffffffu) + hi32(c);
-    r.w[7] = lo32(c);
-    // subtract p once if v >= p, i.e. if v + 19 >= 2^255
-    c = (uint64_t)r.w[0] + 19u;
-    uint32_t s[8];
-    s[0] = lo32(c);
-#pragma unroll
-    for (int i = 1; i < 8; i++) {
-        c = (uint64_t)r.w[i] + hi32(c);
-        s[i] = lo32(c);
-    }
-    uint32_t ge = s[7] >> 31;
-    uint32_t m = 0u - ge;
-#pragma unroll
-    for (int i = 0; i < 7; i++) r.w[i] = (s[i] & m) | (r.w[i] & ~m);
-    r.w[7] = ((s[7] & 0x7fffffffu) & m) | (r.w[7] & ~m);
+// 8 little-endian words of the canonical value.
+SPL_HD void fe_to_words(uint32_t w[8], const fe &a) {
+    fe c;
+    fe_canon(c, a);
+    const uint32_t *h = reinterpret_cast<const uint32_t *>(c.v);
+    w[0] = h[0] | (h[1] << 26);
+    w[1] = (h[1] >> 6) | (h[2] << 19);
+    w[2] = (h[2] >> 13) | (h[3] << 13);
+    w[3] = (h[3] >> 19) | (h[4] << 6);
+    w[4] = h[5] | (h[6] << 25);
+    w[5] = (h[6] >> 7) | (h[7] << 19);
+    w[6] = (h[7] >> 13) | (h[8] << 12);
+    w[7] = (h[8] >> 20) | (h[9] << 6);
+}
+
+// Low 255 bits of 8 little-endian words (bit 255 ignored; a value >= p is
+// kept as-is and reduced by the arithmetic).
+SPL_HD void fe_from_words(fe &r, const uint32_t w[8]) {
+    r.v[0] = (int32_t)(w[0] & 0x3ffffff);
+    r.v[1] = (int32_t)(((w[0] >> 26) | (w[1] << 6)) & 0x1ffffff);
+    r.v[2] = (int32_t)(((w[1] >> 19) | (w[2] << 13)) & 0x3ffffff);
+    r.v[3] = (int32_t)(((w[2] >> 13) | (w[3] << 19)) & 0x1ffffff);
+    r.v[4] = (int32_t)((w[3] >> 6) & 0x3ffffff);
+    r.v[5] = (int32_t)(w[4] & 0x1ffffff);
+    r.v[6] = (int32_t)(((w[4] >> 25) | (w[5] << 7)) & 0x3ffffff);
+    r.v[7] = (int32_t)(((w[5] >> 19) | (w[6] << 13)) & 0x1ffffff);
+    r.v[8] = (int32_t)(((w[6] >> 12) | (w[7] << 20)) & 0x3ffffff);
+    r.v[9] = (int32_t)((w[7] >> 6) & 0x1ffffff);
 }
 
 SPL_HD uint32_t fe_iszero(const fe &a) {
     fe c;
     fe_canon(c, a);
-    uint32_t o = 0;
+    int32_t o = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) o |= c.w[i];
+    for (int i = 0; i < 10; i++) o |= c.v[i];
     return o == 0;
 }
 
 SPL_HD uint32_t fe_isneg(const fe &a) {
     fe c;
     fe_canon(c, a);
-    return c.w[0] & 1;
-}
-
-// bytes -> field element, ignoring bit 255 (non-canonical y >= p is kept).
-SPL_HD void fe_from_words(fe &r, const uint32_t w[8]) {
-#pragma unroll
-    for (int i = 0; i < 7; i++) r.w[i] = w[i];
-    r.w[7] = w[7] & 0x7fffffffu;
-}
-
-SPL_HD void fe_to_words(uint32_t w[8], const fe &a) {
-    fe c;
-    fe_canon(c, a);
-#pragma unroll
-    for (int i = 0; i < 8; i++) w[i] = c.w[i];
-}
-
-SPL_HD void fe_cmov(fe &r, const fe &a, uint32_t flag) {
-    uint32_t m = 0u - flag;
-#pragma unroll
-    for (int i = 0; i < 8; i++) r.w[i] = (r.w[i] & ~m) | (a.w[i] & m);
+    return (uint32_t)c.v[0] & 1u;
 }
 
 // z^(2^250 - 1) and z^11: the common prefix of the inversion and square-root

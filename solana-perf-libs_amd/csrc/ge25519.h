@@ -21,7 +21,7 @@ struct ge_p1p1 { fe X, Y, Z, T; };
 struct ge_precomp { fe yplusx, yminusx, xy2d; };
 struct ge_cached { fe YplusX, YminusX, Z, T2d; };
 
-// Curve constants, as radix-2^32 words (values < p).
+// Curve constants (canonical field elements).
 struct curve_consts {
     fe d;        // -121665/121666
     fe d2;       // 2d
@@ -54,13 +54,13 @@ SPL_HD void ge_p3_to_cached(ge_cached &r, const ge_p3 &p, const fe &d2) {
     fe_mul(r.T2d, p.T, d2);
 }
 
-// r = 2p, p projective: 3 squares + 1 square of (X+Y)
+// r = 2p, p projective: 3 squares (one doubled in the column sums so that
+// T stays a valid multiplier input) + 1 square of (X+Y)
 SPL_HD void ge_p2_dbl(ge_p1p1 &r, const ge_p2 &p) {
     fe xx, yy, b, aa;
     fe_sq(xx, p.X);
     fe_sq(yy, p.Y);
-    fe_sq(b, p.Z);
-    fe_add(b, b, b);
+    fe_sq2(b, p.Z);
     fe_add(aa, p.X, p.Y);
     fe_sq(aa, aa);
     fe_add(r.Y, yy, xx);
@@ -172,29 +172,16 @@ SPL_HD int ge_decode_neg(ge_p3 &h, const uint32_t enc[8], const curve_consts &k)
 // Order-8 points are not flagged.  Input: the 32-byte encoding as words.
 SPL_HD uint32_t y_low_order(const uint32_t enc[8]) {
     fe y;
+    uint32_t w[8];
     fe_from_words(y, enc);
-    fe_canon(y, y);
-    uint32_t mid0 = 0, mid1 = 0;   // OR of words 1..6, AND of words 1..6
-    mid1 = 0xffffffffu;
+    fe_to_words(w, y);
+    uint32_t mid0 = 0, mid1 = 0xffffffffu;   // OR and AND of words 1..6
 #pragma unroll
-    for (int i = 1; i < 7; i++) { mid0 |= y.w[i]; mid1 &= y.w[i]; }
-    uint32_t is0 = (y.w[0] == 0) & (mid0 == 0) & (y.w[7] == 0);
-    uint32_t is1 = (y.w[0] == 1) & (mid0 == 0) & (y.w[7] == 0);
-    uint32_t ispm1 = (y.w[0] == 0xffffffecu) & (mid1 == 0xffffffffu) & (y.w[7] == 0x7fffffffu);
+    for (int i = 1; i < 7; i++) { mid0 |= w[i]; mid1 &= w[i]; }
+    uint32_t is0 = (w[0] == 0) & (mid0 == 0) & (w[7] == 0);
+    uint32_t is1 = (w[0] == 1) & (mid0 == 0) & (w[7] == 0);
+    uint32_t ispm1 = (w[0] == 0xffffffecu) & (mid1 == 0xffffffffu) & (w[7] == 0x7fffffffu);
     return is0 | is1 | ispm1;
-}
-
-// 1 if the 32-byte encoding is y < p with x = 0 and bit 255 clear or x != 0:
-// i.e. what a canonical encoder can output.  Used by verify, where the
-// comparison enc(R') == R_bytes can only succeed for such encodings.
-SPL_HD uint32_t y_is_canonical(const uint32_t enc[8]) {
-    fe y, c;
-    fe_from_words(y, enc);
-    fe_canon(c, y);
-    uint32_t d = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) d |= c.w[i] ^ y.w[i];
-    return d == 0;
 }
 
 }  // namespace spl
