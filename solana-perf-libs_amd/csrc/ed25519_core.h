@@ -15,6 +15,15 @@
 // small-order test reduces to y_low_order(R).  So step 2 is evaluated as
 // "not y_low_order(R)" without decompressing R: same mask, one square root
 // (about 260 field operations) less per signature.
+//
+// Scalar multiplication uses windows at the same bit positions in every lane
+// of a wave: [h](-A) in signed radix-16 digits (an add every 4 bits from a
+// 9-entry table 0..8 x (-A)) and [S]B in signed radix-256 digits (an add every
+// 8 bits from a precomputed 129-entry table 0..128 x B).  A zero digit adds the
+// identity.  Sliding windows, as in the reference (ge.cu:25-58, 134-171), put
+// the adds at data-dependent positions; across 64 lanes nearly every bit
+// position then needs an add in some lane and the whole wave pays for it.
+// The group law is exact, so the result is the same point either way.
 #pragma once
 #include "ge25519.h"
 #include "sc25519.h"
@@ -24,9 +33,22 @@ namespace spl {
 
 struct ed25519_tables {
     curve_consts k;
-    ge_precomp Bi[8];          // (2j+1) B, j = 0..7
-    ge_precomp base[32][8];    // (j+1) 256^i B
+    ge_precomp base[32][8];    // (j+1) 256^i B: signing (constant-time scans)
+    ge_precomp Bw8[129];       // d B for d = 0..128 (entry 0 = identity): verify
 };
+
+// Signed digit j of a scalar in radix 2^w, computed from digit j and the top
+// bit of digit j-1 alone: e_j = n_j - 2^w t_j + t_{j-1}, t_j = top bit of n_j.
+// Sum e_j 2^(w j) = sum n_j 2^(w j) (the t terms telescope), |e_j| <= 2^(w-1);
+// exact as long as the top digit's t is 0.  a: 8 little-endian words.
+template <int W>
+SPL_HD int signed_digit(const uint32_t a[8], int j) {
+    const int per = 32 / W;
+    uint32_t n = (a[j / per] >> (W * (j % per))) & ((1u << W) - 1);
+    uint32_t prev = 0;
+    if (j > 0) prev = (a[(j - 1) / per] >> (W * ((j - 1) % per) + W - 1)) & 1u;
+    return (int)n - (int)((n >> (W - 1)) << W) + (int)prev;
+}
 
 // ------------------------------------------------------------ message reader
 
@@ -106,7 +128,7 @@ SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw
 // ------------------------------------------------------------ verify
 
 // Returns 1 when the signature verifies under the rule above.  Ai is caller
-// storage for the 8 cached odd multiples of -A.
+// storage for the 9-entry table 0..8 x (-A).
 SPL_HD int verify_one(const uint32_t sig[16], const uint32_t pk[8], const MsgReader &msg,
                       const ed25519_tables &T, ge_cached *Ai) {
     if (sig[15] & 0xf0000000u) return 0;           // S >= 2^252
@@ -124,34 +146,47 @@ SPL_HD int verify_one(const uint32_t sig[16], const uint32_t pk[8], const MsgRea
     if (ge_decode_neg(A, pk, T.k) != 0) return 0;
     if (y_low_order(pk)) return 0;
 
+    // Ai[k] = k (-A), k = 0..8, cached form; Ai[0] is the identity
     ge_p1p1 t;
-    ge_p3 A2, u;
-    ge_p3_to_cached(Ai[0], A, T.k.d2);
+    ge_p3 u, A2;
+    fe_set(Ai[0].YplusX, 1); fe_set(Ai[0].YminusX, 1);
+    fe_set(Ai[0].Z, 1); fe_set(Ai[0].T2d, 0);
+    ge_p3_to_cached(Ai[1], A, T.k.d2);
     ge_p3_dbl(t, A);
     ge_p1p1_to_p3(A2, t);
-    for (int j = 1; j < 8; j++) {
-        ge_add_cached(t, A2, Ai[j - 1], 0);
+    ge_p3_to_cached(Ai[2], A2, T.k.d2);
+    for (int k = 3; k <= 8; k++) {
+        ge_add_cached(t, A2, Ai[k - 2], 0);
         ge_p1p1_to_p3(u, t);
-        ge_p3_to_cached(Ai[j], u, T.k.d2);
+        ge_p3_to_cached(Ai[k], u, T.k.d2);
     }
 
-    int8_t as[256], bs[256];
-    sc_slide(as, h);
-    sc_slide(bs, sig + 8);
+    // [h](-A) + [S]B, 64 windows of 4 bits from the top; the radix-256 digit
+    // of S joins at every other window.  h < L < 2^253 and S < 2^252 so both
+    // top carries are 0.
+    const uint32_t *S = sig + 8;
     ge_p2 r;
     ge_p2_0(r);
-    int i = 255;
-    while (i >= 0 && !as[i] && !bs[i]) i--;
-    for (; i >= 0; i--) {
-        ge_p2_dbl(t, r);
-        int da = as[i], db = bs[i];
-        if (da) {
+    for (int j = 63; j >= 0; j--) {
+        const int dv = signed_digit<4>(h, j);
+        const ge_cached e = Ai[dv < 0 ? -dv : dv];
+        if (j != 63) {
+            ge_p2_dbl(t, r);
+            ge_p1p1_to_p2(r, t);
+            ge_p2_dbl(t, r);
+            ge_p1p1_to_p2(r, t);
+            ge_p2_dbl(t, r);
+            ge_p1p1_to_p2(r, t);
+            ge_p2_dbl(t, r);
             ge_p1p1_to_p3(u, t);
-            ge_add_cached(t, u, Ai[(da > 0 ? da : -da) >> 1], da < 0);
+        } else {
+            ge_p3_0(u);
         }
-        if (db) {
+        ge_add_cached(t, u, e, dv < 0);
+        if ((j & 1) == 0) {
+            const int df = signed_digit<8>(S, j >> 1);
             ge_p1p1_to_p3(u, t);
-            ge_add_precomp(t, u, T.Bi[(db > 0 ? db : -db) >> 1], db < 0);
+            ge_add_precomp(t, u, T.Bw8[df < 0 ? -df : df], df < 0);
         }
         ge_p1p1_to_p2(r, t);
     }

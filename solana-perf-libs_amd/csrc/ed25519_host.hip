@@ -141,7 +141,7 @@ int ed25519_verify(const unsigned char *signature, const unsigned char *message,
     bytes_to_words(sig, signature, 16);
     bytes_to_words(pk, public_key, 8);
     MsgReader msg{message, message_len};
-    ge_cached Ai[8];
+    ge_cached Ai[9];
     return verify_one(sig, pk, msg, host_tables(), Ai);
 }
 

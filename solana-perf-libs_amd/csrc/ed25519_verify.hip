@@ -27,7 +27,7 @@ verify_kernel(const uint8_t *__restrict__ packets, uint64_t packets_size,
     load_words(sig, packets + so, 16);
     load_words(pk, packets + po, 8);
     MsgReader msg{packets + mo, (uint32_t)ml};
-    ge_cached Ai[8];
+    ge_cached Ai[9];
     out[i] = (uint8_t)verify_one(sig, pk, msg, *T, Ai);
 }
 

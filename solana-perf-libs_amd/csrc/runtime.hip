@@ -85,14 +85,15 @@ static void compute_tables(ed25519_tables &T) {
     ge_decode_neg(B, enc, T.k);
     fe_neg(B.X, B.X);
     fe_neg(B.T, B.T);
-    ge_p3 B2, acc;
     ge_p1p1 t;
-    ge_p3_dbl(t, B);
-    ge_p1p1_to_p3(B2, t);
-    acc = B;
-    for (int j = 0; j < 8; j++) {
-        p3_to_precomp(T.Bi[j], acc, T.k.d2);
-        p3_add(acc, acc, B2, T.k.d2);
+    // Bw8[d] = d B, d = 0..128
+    fe_set(T.Bw8[0].yplusx, 1);
+    fe_set(T.Bw8[0].yminusx, 1);
+    fe_set(T.Bw8[0].xy2d, 0);
+    ge_p3 acc = B;
+    for (int d = 1; d <= 128; d++) {
+        p3_to_precomp(T.Bw8[d], acc, T.k.d2);
+        p3_add(acc, acc, B, T.k.d2);
     }
     ge_p3 row = B;
     for (int i = 0; i < 32; i++) {

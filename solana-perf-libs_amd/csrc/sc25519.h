@@ -95,30 +95,4 @@ SPL_HD void sc_muladd(uint32_t r[8], const uint32_t a[8], const uint32_t b[8],
     sc_reduce512(r, t);
 }
 
-// Signed sliding-window recoding with the reference's rule (ge.cu:25-58):
-// digits odd with |d| <= 15, greedy over a 6-bit look-ahead.  r has 256
-// entries; a is a 256-bit little-endian scalar.
-SPL_HD void sc_slide(int8_t r[256], const uint32_t a[8]) {
-    for (int i = 0; i < 256; i++) r[i] = (int8_t)((a[i >> 5] >> (i & 31)) & 1);
-    for (int i = 0; i < 256; i++) {
-        if (!r[i]) continue;
-        for (int b = 1; b <= 6 && i + b < 256; b++) {
-            if (!r[i + b]) continue;
-            int hi = r[i + b] << b;
-            if (r[i] + hi <= 15) {
-                r[i] = (int8_t)(r[i] + hi);
-                r[i + b] = 0;
-            } else if (r[i] - hi >= -15) {
-                r[i] = (int8_t)(r[i] - hi);
-                for (int k = i + b; k < 256; k++) {
-                    if (!r[k]) { r[k] = 1; break; }
-                    r[k] = 0;
-                }
-            } else {
-                break;
-            }
-        }
-    }
-}
-
 }  // namespace spl
