@@ -126,6 +126,34 @@ int ed25519_sign_many_device(const uint8_t *d_packets, uint64_t packets_size,
 int poh_verify_many_device(uint8_t *d_hashes, const uint64_t *d_num_hashes, size_t num_elems,
                            void *stream);
 
+/* ------------------------------------------------------------------------
+ * Extension: verify engine (no counterpart in the reference).
+ * A verify engine owns device buffers for up to max_signatures signatures and
+ * max_packet_bytes of packets, a HIP stream, and a hipGraph of the tuned
+ * launch sequence (a persistent kernel fed by a device-side work queue, or a
+ * one-shot kernel; both read the signature count from device memory, so one
+ * captured graph serves every batch size up to the capacity).  The caller
+ * writes packets and the four offset tables into the buffers returned by
+ * spl_verify_engine_buffers (same layout as ed25519_verify_many_device),
+ * calls spl_verify_engine_run, then spl_verify_engine_sync and reads the mask.
+ * spl_verify_engine_autotune times every launch configuration (one-shot or
+ * persistent, window width 4/5, register target, length bucketing, waves per
+ * CU) on the batch currently in the buffers, keeps the fastest and records it
+ * for that batch-size class on the device, where the ABI calls pick it up.
+ * An engine is used by one thread at a time.  Functions returning int: 0 ok,
+ * 2 bad arguments, 3 device error. */
+typedef struct spl_verify_engine spl_verify_engine;
+spl_verify_engine *spl_verify_engine_create(uint32_t max_signatures, uint64_t max_packet_bytes);
+void spl_verify_engine_destroy(spl_verify_engine *engine);
+int spl_verify_engine_buffers(spl_verify_engine *engine, uint8_t **d_packets, uint32_t **d_lens,
+                              uint32_t **d_public_key_offsets, uint32_t **d_signature_offsets,
+                              uint32_t **d_message_start_offsets, uint8_t **d_out, void **stream);
+int spl_verify_engine_run(spl_verify_engine *engine, uint32_t num_signatures);
+int spl_verify_engine_sync(spl_verify_engine *engine);
+int spl_verify_engine_autotune(spl_verify_engine *engine, uint32_t num_signatures, char *report,
+                               size_t report_len);
+int spl_verify_engine_config(spl_verify_engine *engine, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

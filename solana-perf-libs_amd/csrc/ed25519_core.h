@@ -17,13 +17,15 @@
 // (about 260 field operations) less per signature.
 //
 // Scalar multiplication uses windows at the same bit positions in every lane
-// of a wave: [h](-A) in signed radix-16 digits (an add every 4 bits from a
-// 9-entry table 0..8 x (-A)) and [S]B in signed radix-256 digits (an add every
-// 8 bits from a precomputed 129-entry table 0..128 x B).  A zero digit adds the
-// identity.  Sliding windows, as in the reference (ge.cu:25-58, 134-171), put
-// the adds at data-dependent positions; across 64 lanes nearly every bit
-// position then needs an add in some lane and the whole wave pays for it.
-// The group law is exact, so the result is the same point either way.
+// of a wave: [h](-A) in signed radix-2^W digits (W = 4 or 5; an add every W
+// bits from a (2^(W-1)+1)-entry table 0..2^(W-1) x (-A)), and [S]B as a comb
+// with no doublings: S's 32 signed radix-256 digits each select from a
+// precomputed table d 256^m B (d = 0..128), accumulated in a second point that
+// is independent of the -A chain (instruction-level parallelism).  A zero digit
+// adds the identity.  Sliding windows, as in the reference (ge.cu:25-58,
+// 134-171), put the adds at data-dependent positions; across 64 lanes nearly
+// every bit position then needs an add in some lane and the whole wave pays
+// for it.  The group law is exact, so the result is the same point either way.
 #pragma once
 #include "ge25519.h"
 #include "sc25519.h"
@@ -33,22 +35,48 @@ namespace spl {
 
 struct ed25519_tables {
     curve_consts k;
-    ge_precomp base[32][8];    // (j+1) 256^i B: signing (constant-time scans)
-    ge_precomp Bw8[129];       // d B for d = 0..128 (entry 0 = identity): verify
+    ge_precomp base[32][8];     // (j+1) 256^i B: signing (constant-time scans)
+    ge_precomp comb[32][129];   // d 256^m B, d = 0..128 (entry 0 = identity): verify
 };
 
-// Signed digit j of a scalar in radix 2^w, computed from digit j and the top
-// bit of digit j-1 alone: e_j = n_j - 2^w t_j + t_{j-1}, t_j = top bit of n_j.
-// Sum e_j 2^(w j) = sum n_j 2^(w j) (the t terms telescope), |e_j| <= 2^(w-1);
-// exact as long as the top digit's t is 0.  a: 8 little-endian words.
+// Signed digits.  Digit j of a scalar in radix 2^W is e_j = n_j - 2^W t_j +
+// t_{j-1}, where n_j is the unsigned digit and t_j its top bit:
+// sum e_j 2^(W j) = sum n_j 2^(W j) (the t terms telescope), |e_j| <= 2^(W-1),
+// exact as long as the top digit's t is 0.  Each digit needs only its own bits
+// and the bit just below them, so digits are produced on the fly.
+
+// Consumes digits from the top: the scalar is held left-aligned in 8 words;
+// each call returns the top digit and shifts the scalar left by W bits.
 template <int W>
-SPL_HD int signed_digit(const uint32_t a[8], int j) {
-    const int per = 32 / W;
-    uint32_t n = (a[j / per] >> (W * (j % per))) & ((1u << W) - 1);
-    uint32_t prev = 0;
-    if (j > 0) prev = (a[(j - 1) / per] >> (W * ((j - 1) % per) + W - 1)) & 1u;
-    return (int)n - (int)((n >> (W - 1)) << W) + (int)prev;
+SPL_HD int take_top_digit(uint32_t a[8]) {
+    const uint32_t n = a[7] >> (32 - W);
+    const uint32_t below = (a[7] >> (31 - W)) & 1u;
+#pragma unroll
+    for (int i = 7; i > 0; i--) a[i] = (a[i] << W) | (a[i - 1] >> (32 - W));
+    a[0] <<= W;
+    return (int)n - (int)((n >> (W - 1)) << W) + (int)below;
 }
+
+// Consumes radix-256 digits from the bottom; `below` carries the previous
+// byte's top bit between calls (start with 0).
+SPL_HD int take_low_byte_digit(uint32_t a[8], uint32_t &below) {
+    const uint32_t n = a[0] & 0xffu;
+    const int d = (int)n - (int)((n >> 7) << 8) + (int)below;
+    below = n >> 7;
+#pragma unroll
+    for (int i = 0; i < 7; i++) a[i] = (a[i] >> 8) | (a[i + 1] << 24);
+    a[7] >>= 8;
+    return d;
+}
+
+// Variable-base window geometry for [h](-A) with h < 2^253.
+template <int W>
+struct VarWindow {
+    static constexpr int kWindows = (253 + W - 1) / W;   // digits 0..kWindows-1
+    static constexpr int kTable = (1 << (W - 1)) + 1;    // entries 0..2^(W-1)
+    static constexpr int kAlign = 32 * 8 - W * kWindows; // left shift to top-align
+    static_assert(W * kWindows - 1 >= 253, "top digit must carry nothing out");
+};
 
 // ------------------------------------------------------------ message reader
 
@@ -127,10 +155,11 @@ SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw
 
 // ------------------------------------------------------------ verify
 
-// Returns 1 when the signature verifies under the rule above.  Ai is caller
-// storage for the 9-entry table 0..8 x (-A).
-SPL_HD int verify_one(const uint32_t sig[16], const uint32_t pk[8], const MsgReader &msg,
-                      const ed25519_tables &T, ge_cached *Ai) {
+// Stage 1 of verify: every check that does not need the scalar
+// multiplication, plus h = SHA-512(R || A || M) mod L and the decoded -A.
+// Returns 0 when the signature is already rejected.
+SPL_HD int verify_prepare(const uint32_t sig[16], const uint32_t pk[8], const MsgReader &msg,
+                          const ed25519_tables &T, uint32_t h[8], ge_p3 &A) {
     if (sig[15] & 0xf0000000u) return 0;           // S >= 2^252
     if (y_low_order(sig)) return 0;                // R of order <= 4
     uint32_t pre[16];
@@ -138,64 +167,97 @@ SPL_HD int verify_one(const uint32_t sig[16], const uint32_t pk[8], const MsgRea
     for (int i = 0; i < 8; i++) { pre[i] = sig[i]; pre[8 + i] = pk[i]; }
     uint64_t st[8];
     sha512_prefixed(st, pre, 16, msg);
-    uint32_t hw[16], h[8];
+    uint32_t hw[16];
     sha512_digest_le_words(hw, st);
     sc_reduce512(h, hw);
-
-    ge_p3 A;
     if (ge_decode_neg(A, pk, T.k) != 0) return 0;
     if (y_low_order(pk)) return 0;
+    return 1;
+}
 
-    // Ai[k] = k (-A), k = 0..8, cached form; Ai[0] is the identity
+// Ai[k] = k A for k = 0..n-1 in cached form (Ai[0] is the identity).
+SPL_HD void build_multiples(ge_cached *Ai, int n, const ge_p3 &A, const fe &d2) {
     ge_p1p1 t;
     ge_p3 u, A2;
     fe_set(Ai[0].YplusX, 1); fe_set(Ai[0].YminusX, 1);
     fe_set(Ai[0].Z, 1); fe_set(Ai[0].T2d, 0);
-    ge_p3_to_cached(Ai[1], A, T.k.d2);
+    ge_p3_to_cached(Ai[1], A, d2);
     ge_p3_dbl(t, A);
     ge_p1p1_to_p3(A2, t);
-    ge_p3_to_cached(Ai[2], A2, T.k.d2);
-    for (int k = 3; k <= 8; k++) {
+    ge_p3_to_cached(Ai[2], A2, d2);
+    for (int k = 3; k < n; k++) {
         ge_add_cached(t, A2, Ai[k - 2], 0);
         ge_p1p1_to_p3(u, t);
-        ge_p3_to_cached(Ai[k], u, T.k.d2);
+        ge_p3_to_cached(Ai[k], u, d2);
     }
+}
 
-    // [h](-A) + [S]B, 64 windows of 4 bits from the top; the radix-256 digit
-    // of S joins at every other window.  h < L < 2^253 and S < 2^252 so both
-    // top carries are 0.
-    const uint32_t *S = sig + 8;
+// Stage 2: R' = [h](-A) + [S]B with the table Ai = 0..2^(W-1) x (-A); returns
+// 1 when encode(R') equals the R bytes of the signature.
+template <int W>
+SPL_HD int verify_finish(const uint32_t sig[16], const uint32_t h[8], const ge_cached *Ai,
+                         const ed25519_tables &T) {
+    typedef VarWindow<W> G;
+    uint32_t a[8], s[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) { a[i] = h[i]; s[i] = sig[8 + i]; }
+    if (G::kAlign) {
+#pragma unroll
+        for (int i = 7; i > 0; i--) a[i] = (a[i] << G::kAlign) | (a[i - 1] >> (32 - G::kAlign));
+        a[0] <<= G::kAlign;
+    }
+    uint32_t sbelow = 0;
+    ge_p3 pb, u;
+    ge_p3_0(pb);
+    ge_p1p1 t, tb;
     ge_p2 r;
     ge_p2_0(r);
-    for (int j = 63; j >= 0; j--) {
-        const int dv = signed_digit<4>(h, j);
+    for (int j = G::kWindows - 1; j >= 0; j--) {
+        const int dv = take_top_digit<W>(a);
         const ge_cached e = Ai[dv < 0 ? -dv : dv];
-        if (j != 63) {
-            ge_p2_dbl(t, r);
-            ge_p1p1_to_p2(r, t);
-            ge_p2_dbl(t, r);
-            ge_p1p1_to_p2(r, t);
-            ge_p2_dbl(t, r);
-            ge_p1p1_to_p2(r, t);
+        if (j < 32) {
+            // comb digit m = 31 - j of S (S < 2^252: the last digit carries 0)
+            const int df = take_low_byte_digit(s, sbelow);
+            ge_add_precomp(tb, pb, T.comb[31 - j][df < 0 ? -df : df], df < 0);
+            ge_p1p1_to_p3(pb, tb);
+        }
+        if (j != G::kWindows - 1) {
+#pragma unroll
+            for (int k = 0; k < W - 1; k++) {
+                ge_p2_dbl(t, r);
+                ge_p1p1_to_p2(r, t);
+            }
             ge_p2_dbl(t, r);
             ge_p1p1_to_p3(u, t);
         } else {
             ge_p3_0(u);
         }
         ge_add_cached(t, u, e, dv < 0);
-        if ((j & 1) == 0) {
-            const int df = signed_digit<8>(S, j >> 1);
-            ge_p1p1_to_p3(u, t);
-            ge_add_precomp(t, u, T.Bw8[df < 0 ? -df : df], df < 0);
-        }
-        ge_p1p1_to_p2(r, t);
+        if (j != 0) ge_p1p1_to_p2(r, t);
     }
+    ge_cached cb;
+    ge_p1p1_to_p3(u, t);
+    ge_p3_to_cached(cb, pb, T.k.d2);
+    ge_add_cached(t, u, cb, 0);
+    ge_p1p1_to_p2(r, t);
     uint32_t enc[8];
     ge_encode(enc, r.X, r.Y, r.Z);
     uint32_t diff = 0;
 #pragma unroll
     for (int j = 0; j < 8; j++) diff |= enc[j] ^ sig[j];
     return diff == 0;
+}
+
+// Returns 1 when the signature verifies under the rule above.  Ai is caller
+// storage for VarWindow<W>::kTable cached points.
+template <int W = 4>
+SPL_HD int verify_one(const uint32_t sig[16], const uint32_t pk[8], const MsgReader &msg,
+                      const ed25519_tables &T, ge_cached *Ai) {
+    uint32_t h[8];
+    ge_p3 A;
+    if (!verify_prepare(sig, pk, msg, T, h, A)) return 0;
+    build_multiples(Ai, VarWindow<W>::kTable, A, T.k.d2);
+    return verify_finish<W>(sig, h, Ai, T);
 }
 
 // ------------------------------------------------------------ fixed-base mult

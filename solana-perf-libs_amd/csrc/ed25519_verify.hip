@@ -1,45 +1,47 @@
-// Batched Ed25519 verification behind the reference's ed25519_verify_many ABI.
+// Batched Ed25519 verification: the reference's ed25519_verify_many ABI, the
+// device-resident entry point, and the verify engine (graph-captured
+// persistent launch + autotuner).
 //
-// Reference: src/cuda-ecc-ed25519/verify.cu:176-340 — three kernels
-// (verify_kernel: S check, R check, SHA-512, sc_reduce; gen_lookup_kernel:
-// decode A, build the 8-entry table in global memory; scalar_double_kernel:
-// double-scalar multiplication and compare) with a per-signature mask threaded
-// through global memory.  Here one fused kernel does the whole check per
-// signature; the odd-multiple table of -A lives in the lane's private segment.
-#include "runtime.h"
+// Reference: src/cuda-ecc-ed25519/verify.cu:240-340 (ABI, three launches),
+// gpu_ctx.cu:75-144 (per-elem H2D ingest).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "verify_kernels.h"
 #include "../../include/cuda_crypt.h"
 
 namespace spl {
 
-__global__ void __launch_bounds__(64)
-verify_kernel(const uint8_t *__restrict__ packets, uint64_t packets_size,
-              const uint32_t *__restrict__ lens, const uint32_t *__restrict__ pk_off,
-              const uint32_t *__restrict__ sig_off, const uint32_t *__restrict__ msg_off,
-              uint32_t n, uint8_t *__restrict__ out, const ed25519_tables *__restrict__ T) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint64_t so = sig_off[i], po = pk_off[i], mo = msg_off[i], ml = lens[i];
-    if (so + 64 > packets_size || po + 32 > packets_size || mo + ml > packets_size) {
-        out[i] = 0;    // offsets outside the packet buffer: reject
-        return;
+// ------------------------------------------------------------ tuning cache
+
+// Best configuration per (device, size class = bit length of n), filled by
+// the autotuner; the ABI and device entry points use the best one-shot
+// configuration of the class, the engine uses the overall best.
+struct TuneEntry {
+    bool valid = false;
+    VerifyConfig best, best_plain;
+};
+static std::mutex g_tune_mu;
+static TuneEntry g_tune[kMaxDevices][33];
+
+static int size_class(uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; }
+
+static VerifyConfig tuned_plain_config(int dev, uint32_t n) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    if (dev >= 0 && dev < kMaxDevices) {
+        const TuneEntry &e = g_tune[dev][size_class(n)];
+        if (e.valid) return e.best_plain;
     }
-    uint32_t sig[16], pk[8];
-    load_words(sig, packets + so, 16);
-    load_words(pk, packets + po, 8);
-    MsgReader msg{packets + mo, (uint32_t)ml};
-    ge_cached Ai[9];
-    out[i] = (uint8_t)verify_one(sig, pk, msg, *T, Ai);
+    return kDefaultVerifyConfig;
 }
 
-hipError_t launch_verify(const uint8_t *d_packets, uint64_t packets_size, const uint32_t *d_lens,
-                         const uint32_t *d_pk, const uint32_t *d_sig, const uint32_t *d_msg,
-                         uint32_t n, uint8_t *d_out, const ed25519_tables *d_tables,
-                         hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const uint32_t block = 64;
-    hipLaunchKernelGGL(verify_kernel, dim3((n + block - 1) / block), dim3(block), 0, s,
-                       d_packets, packets_size, d_lens, d_pk, d_sig, d_msg, n, d_out, d_tables);
-    return hipGetLastError();
+static int device_cus(int dev) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return cus;
 }
 
 // Packs gpu_Elems and the four offset tables into the queue's pinned staging
@@ -75,7 +77,80 @@ hipError_t stage_packets(QueueCtx &q, const gpu_Elems *elems, uint32_t num_elems
     return hipMemcpyAsync(q.d_offsets.ptr, tab, tab_bytes, hipMemcpyHostToDevice, s);
 }
 
+VerifyJob make_verify_job(const uint8_t *packets, uint64_t size, const uint32_t *lens,
+                          const uint32_t *pk, const uint32_t *sig, const uint32_t *msg,
+                          uint8_t *out, void *scratch, const ed25519_tables *T) {
+    VerifyJob j{};
+    j.packets = packets;
+    j.packets_size = size;
+    j.lens = lens;
+    j.pk_off = pk;
+    j.sig_off = sig;
+    j.msg_off = msg;
+    j.out = out;
+    j.scratch = static_cast<ge_cached *>(scratch);
+    j.T = T;
+    return j;
+}
+
 }  // namespace spl
+
+// ------------------------------------------------------------ verify engine
+
+// Device buffers sized for a capacity, one stream, and one instantiated graph
+// of the tuned launch sequence.  Inputs are written by the caller straight
+// into the engine's buffers; run() sets the signature count in device memory
+// (outside the graph) and replays the graph.
+struct spl_verify_engine {
+    int dev = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    uint32_t cap_sigs = 0;
+    uint64_t cap_bytes = 0;
+    spl::Buf packets, tabs, out, scratch, work, order, n_dev;
+    spl::VerifyConfig cfg = spl::kDefaultVerifyConfig;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const spl::ed25519_tables *T = nullptr;
+
+    spl::VerifyJob job() const {
+        const uint32_t *t = tabs.as<uint32_t>();
+        spl::VerifyJob j = spl::make_verify_job(packets.as<uint8_t>(), cap_bytes, t,
+                                                t + cap_sigs, t + 2ull * cap_sigs,
+                                                t + 3ull * cap_sigs, out.as<uint8_t>(),
+                                                scratch.ptr, T);
+        j.n_dev = n_dev.as<uint32_t>();
+        j.work = work.as<uint32_t>();
+        j.order = order.as<uint32_t>();
+        return j;
+    }
+
+    hipError_t ensure_scratch(const spl::VerifyConfig &c) {
+        return scratch.reserve(spl::verify_scratch_bytes(c, cap_sigs, cus));
+    }
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+
+    // Captures the launch sequence of `cfg` (grid sized for the capacity; the
+    // kernels read the count from n_dev).
+    hipError_t capture() {
+        drop_graph();
+        hipError_t e = ensure_scratch(cfg);
+        if (e != hipSuccess) return e;
+        e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t le = spl::launch_verify_cfg(cfg, job(), cap_sigs, cus, stream);
+        e = hipStreamEndCapture(stream, &graph);
+        if (le != hipSuccess) return le;
+        if (e != hipSuccess) return e;
+        return hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    }
+};
 
 using namespace spl;
 
@@ -105,12 +180,17 @@ void ed25519_verify_many(const gpu_Elems *elems, uint32_t num_elems, uint32_t me
     uint64_t pbytes = (uint64_t)total_packets * message_size;
     const uint32_t *tab = nullptr;
     uint8_t *hout = nullptr;
+    const VerifyConfig cfg = tuned_plain_config(l.dev->id, n);
+    VerifyJob job;
     SPL_HIP_TRY(stage_packets(q, elems, num_elems, message_size, pbytes, n, message_lens,
                               public_key_offsets, signature_offsets, message_start_offsets, s));
     SPL_HIP_TRY(q.d_out.reserve(n));
+    SPL_HIP_TRY(q.d_scratch.reserve(verify_scratch_bytes(cfg, n, 0)));
     tab = q.d_offsets.as<uint32_t>();
-    SPL_HIP_TRY(launch_verify(q.d_packets.as<uint8_t>(), pbytes, tab, tab + n, tab + 2ull * n,
-                              tab + 3ull * n, n, q.d_out.as<uint8_t>(), l.dev->d_tables, s));
+    job = make_verify_job(q.d_packets.as<uint8_t>(), pbytes, tab, tab + n, tab + 2ull * n,
+                          tab + 3ull * n, q.d_out.as<uint8_t>(), q.d_scratch.ptr,
+                          l.dev->d_tables);
+    SPL_HIP_TRY(launch_verify_cfg(cfg, job, n, 0, s));
     hout = q.h_stage.as<uint8_t>();
     SPL_HIP_TRY(hipMemcpyAsync(hout, q.d_out.ptr, n, hipMemcpyDeviceToHost, s));
     SPL_HIP_TRY(hipStreamSynchronize(s));
@@ -133,13 +213,167 @@ int ed25519_verify_many_device(const uint8_t *d_packets, uint64_t packets_size,
     if (hipGetDevice(&dev) != hipSuccess) return 1;
     const ed25519_tables *tabs = device_tables(dev);
     if (!tabs) return 1;
-    hipError_t e = launch_verify(d_packets, packets_size, d_message_lens, d_public_key_offsets,
-                                 d_signature_offsets, d_message_start_offsets, total_signatures,
-                                 d_out, tabs, (hipStream_t)stream);
+    const VerifyConfig cfg = tuned_plain_config(dev, total_signatures);
+    void *scr = stream_scratch(dev, (hipStream_t)stream,
+                               verify_scratch_bytes(cfg, total_signatures, 0));
+    if (!scr) {
+        fprintf(stderr, "ed25519_verify_many_device: scratch allocation failed\n");
+        return 3;
+    }
+    VerifyJob job = make_verify_job(d_packets, packets_size, d_message_lens,
+                                    d_public_key_offsets, d_signature_offsets,
+                                    d_message_start_offsets, d_out, scr, tabs);
+    hipError_t e = launch_verify_cfg(cfg, job, total_signatures, 0, (hipStream_t)stream);
     if (e != hipSuccess) {
         fprintf(stderr, "ed25519_verify_many_device: launch failed: %s\n", hipGetErrorString(e));
         return 3;
     }
+    return 0;
+}
+
+spl_verify_engine *spl_verify_engine_create(uint32_t max_signatures, uint64_t max_packet_bytes) {
+    if (max_signatures == 0 || max_packet_bytes == 0) return nullptr;
+    if (pool_init() <= 0) return nullptr;
+    auto *eng = new spl_verify_engine();
+    bool ok = hipGetDevice(&eng->dev) == hipSuccess;
+    if (ok) eng->T = device_tables(eng->dev);
+    ok = ok && eng->T;
+    if (ok) {
+        eng->cus = device_cus(eng->dev);
+        eng->cap_sigs = max_signatures;
+        eng->cap_bytes = max_packet_bytes;
+        ok = hipStreamCreate(&eng->stream) == hipSuccess &&
+             eng->packets.reserve(max_packet_bytes + 16) == hipSuccess &&
+             eng->tabs.reserve(16ull * max_signatures) == hipSuccess &&
+             eng->out.reserve(max_signatures) == hipSuccess &&
+             eng->work.reserve(64 * 4) == hipSuccess &&
+             eng->order.reserve(4ull * max_signatures) == hipSuccess &&
+             eng->n_dev.reserve(4) == hipSuccess;
+    }
+    if (ok) {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        const TuneEntry &t = g_tune[eng->dev][size_class(max_signatures)];
+        if (t.valid) eng->cfg = t.best;
+    }
+    ok = ok && eng->capture() == hipSuccess;
+    if (!ok) {
+        fprintf(stderr, "spl_verify_engine_create: failed\n");
+        spl_verify_engine_destroy(eng);
+        return nullptr;
+    }
+    return eng;
+}
+
+void spl_verify_engine_destroy(spl_verify_engine *eng) {
+    if (!eng) return;
+    if (eng->stream) (void)hipStreamSynchronize(eng->stream);
+    eng->drop_graph();
+    eng->packets.release();
+    eng->tabs.release();
+    eng->out.release();
+    eng->scratch.release();
+    eng->work.release();
+    eng->order.release();
+    eng->n_dev.release();
+    if (eng->stream) (void)hipStreamDestroy(eng->stream);
+    delete eng;
+}
+
+int spl_verify_engine_buffers(spl_verify_engine *eng, uint8_t **d_packets, uint32_t **d_lens,
+                              uint32_t **d_pk_off, uint32_t **d_sig_off, uint32_t **d_msg_off,
+                              uint8_t **d_out, void **stream) {
+    if (!eng) return 2;
+    uint32_t *t = eng->tabs.as<uint32_t>();
+    if (d_packets) *d_packets = eng->packets.as<uint8_t>();
+    if (d_lens) *d_lens = t;
+    if (d_pk_off) *d_pk_off = t + eng->cap_sigs;
+    if (d_sig_off) *d_sig_off = t + 2ull * eng->cap_sigs;
+    if (d_msg_off) *d_msg_off = t + 3ull * eng->cap_sigs;
+    if (d_out) *d_out = eng->out.as<uint8_t>();
+    if (stream) *stream = eng->stream;
+    return 0;
+}
+
+int spl_verify_engine_run(spl_verify_engine *eng, uint32_t n) {
+    if (!eng) return 2;
+    if (n > eng->cap_sigs) return 2;
+    if (n == 0) return 0;
+    launch_set_n(eng->n_dev.as<uint32_t>(), n, eng->stream);
+    hipError_t e = hipGraphLaunch(eng->exec, eng->stream);
+    if (e != hipSuccess) {
+        fprintf(stderr, "spl_verify_engine_run: %s\n", hipGetErrorString(e));
+        return 3;
+    }
+    return 0;
+}
+
+int spl_verify_engine_sync(spl_verify_engine *eng) {
+    if (!eng) return 2;
+    return hipStreamSynchronize(eng->stream) == hipSuccess ? 0 : 3;
+}
+
+int spl_verify_engine_config(spl_verify_engine *eng, char *buf, size_t len) {
+    if (!eng || !buf || !len) return 2;
+    snprintf(buf, len, "%s W=%u waves_per_eu=%u sort=%u waves_per_cu=%u",
+             eng->cfg.persistent ? "persistent" : "one-shot", eng->cfg.window,
+             eng->cfg.waves_per_eu, eng->cfg.sort, eng->cfg.waves_per_cu);
+    return 0;
+}
+
+// Times every candidate configuration on the batch currently in the engine's
+// buffers (n signatures), keeps the fastest for this engine (recapturing its
+// graph) and records it for n's size class on this device.
+int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
+                               size_t report_len) {
+    if (!eng || n == 0 || n > eng->cap_sigs) return 2;
+    VerifyConfig cand[32];
+    const int nc = verify_config_candidates(cand, 32);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 3;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return 3;
+    }
+    float best_ms = 1e30f, best_plain_ms = 1e30f;
+    VerifyConfig best = eng->cfg, best_plain = kDefaultVerifyConfig;
+    std::string rep;
+    int rc = 0;
+    for (int i = 0; i < nc && rc == 0; i++) {
+        const VerifyConfig &c = cand[i];
+        if (eng->ensure_scratch(c) != hipSuccess) continue;
+        VerifyJob j = eng->job();
+        float ms = 0;
+        for (int r = 0; r < 2 && rc == 0; r++) {   // the first run warms the caches
+            launch_set_n(const_cast<uint32_t *>(j.n_dev), n, eng->stream);
+            if (hipEventRecord(e0, eng->stream) != hipSuccess ||
+                launch_verify_cfg(c, j, n, eng->cus, eng->stream) != hipSuccess ||
+                hipEventRecord(e1, eng->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                rc = 3;
+        }
+        if (rc) break;
+        char line[160];
+        snprintf(line, sizeof line, "%s W=%u wpe=%u sort=%u wpc=%u: %.3f ms\n",
+                 c.persistent ? "persistent" : "one-shot", c.window, c.waves_per_eu, c.sort,
+                 c.waves_per_cu, ms);
+        rep += line;
+        if (ms < best_ms) { best_ms = ms; best = c; }
+        if (!c.persistent && ms < best_plain_ms) { best_plain_ms = ms; best_plain = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    eng->cfg = best;
+    if (eng->capture() != hipSuccess) return 3;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        TuneEntry &t = g_tune[eng->dev][size_class(n)];
+        t.valid = true;
+        t.best = best;
+        t.best_plain = best_plain;
+    }
+    if (report && report_len) snprintf(report, report_len, "%s", rep.c_str());
     return 0;
 }
 

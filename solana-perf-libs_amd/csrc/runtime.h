@@ -87,6 +87,12 @@ void release_queue(const Lease &l);
 // device is not in the pool).
 const ed25519_tables *device_tables(int hip_device);
 
+// Device scratch for launches enqueued by the device-resident entry points:
+// one grow-only buffer per (device, stream), so launches on one stream reuse
+// it in stream order and launches on different streams never share it.
+// Returns nullptr on allocation failure.
+void *stream_scratch(int hip_device, hipStream_t stream, size_t bytes);
+
 // Frees every cached device/pinned buffer (ed25519_free_gpu_mem).
 void pool_free_buffers();
 

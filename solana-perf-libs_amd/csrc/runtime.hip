@@ -3,6 +3,8 @@
 
 #include <string.h>
 
+#include <vector>
+
 namespace spl {
 
 bool g_verbose = false;
@@ -44,20 +46,36 @@ static void p3_add(ge_p3 &r, const ge_p3 &p, const ge_p3 &q, const fe &d2) {
     ge_p1p1_to_p3(r, t);
 }
 
-static void p3_to_precomp(ge_precomp &r, const ge_p3 &p, const fe &d2) {
-    fe zi, x, y;
-    fe_invert(zi, p.Z);
-    fe_mul(x, p.X, zi);
-    fe_mul(y, p.Y, zi);
-    fe_canon(x, x);
-    fe_canon(y, y);
-    fe_add(r.yplusx, y, x);
-    fe_sub(r.yminusx, y, x);
-    fe_mul(r.xy2d, x, y);
-    fe_mul(r.xy2d, r.xy2d, d2);
-    fe_canon(r.yplusx, r.yplusx);
-    fe_canon(r.yminusx, r.yminusx);
-    fe_canon(r.xy2d, r.xy2d);
+// Affine (y+x, y-x, 2dxy) forms of n points with one field inversion
+// (Montgomery's simultaneous-inversion trick).
+static void batch_to_precomp(ge_precomp *out, const ge_p3 *in, int n, const fe &d2) {
+    std::vector<fe> prefix(n);
+    prefix[0] = in[0].Z;
+    for (int i = 1; i < n; i++) fe_mul(prefix[i], prefix[i - 1], in[i].Z);
+    fe inv;
+    fe_invert(inv, prefix[n - 1]);
+    for (int i = n - 1; i >= 0; i--) {
+        fe zi;
+        if (i > 0) {
+            fe_mul(zi, inv, prefix[i - 1]);
+            fe_mul(inv, inv, in[i].Z);
+        } else {
+            zi = inv;
+        }
+        fe x, y;
+        fe_mul(x, in[i].X, zi);
+        fe_mul(y, in[i].Y, zi);
+        fe_canon(x, x);
+        fe_canon(y, y);
+        ge_precomp &r = out[i];
+        fe_add(r.yplusx, y, x);
+        fe_sub(r.yminusx, y, x);
+        fe_mul(r.xy2d, x, y);
+        fe_mul(r.xy2d, r.xy2d, d2);
+        fe_canon(r.yplusx, r.yplusx);
+        fe_canon(r.yminusx, r.yminusx);
+        fe_canon(r.xy2d, r.xy2d);
+    }
 }
 
 static void compute_tables(ed25519_tables &T) {
@@ -85,27 +103,30 @@ static void compute_tables(ed25519_tables &T) {
     ge_decode_neg(B, enc, T.k);
     fe_neg(B.X, B.X);
     fe_neg(B.T, B.T);
-    ge_p1p1 t;
-    // Bw8[d] = d B, d = 0..128
-    fe_set(T.Bw8[0].yplusx, 1);
-    fe_set(T.Bw8[0].yminusx, 1);
-    fe_set(T.Bw8[0].xy2d, 0);
-    ge_p3 acc = B;
-    for (int d = 1; d <= 128; d++) {
-        p3_to_precomp(T.Bw8[d], acc, T.k.d2);
-        p3_add(acc, acc, B, T.k.d2);
-    }
+    // comb[m][d] = d 256^m B for d = 1..128; comb[m][0] = identity
+    std::vector<ge_p3> pts(32 * 128);
     ge_p3 row = B;
-    for (int i = 0; i < 32; i++) {
-        ge_p3 m = row;
-        for (int j = 0; j < 8; j++) {
-            p3_to_precomp(T.base[i][j], m, T.k.d2);
-            p3_add(m, m, row, T.k.d2);
+    ge_p1p1 t;
+    for (int m = 0; m < 32; m++) {
+        ge_p3 acc = row;
+        for (int d = 1; d <= 128; d++) {
+            pts[m * 128 + d - 1] = acc;
+            p3_add(acc, acc, row, T.k.d2);
         }
         for (int k = 0; k < 8; k++) {
             ge_p3_dbl(t, row);
             ge_p1p1_to_p3(row, t);
         }
+    }
+    std::vector<ge_precomp> pre(32 * 128);
+    batch_to_precomp(pre.data(), pts.data(), 32 * 128, T.k.d2);
+    for (int m = 0; m < 32; m++) {
+        fe_set(T.comb[m][0].yplusx, 1);
+        fe_set(T.comb[m][0].yminusx, 1);
+        fe_set(T.comb[m][0].xy2d, 0);
+        for (int d = 1; d <= 128; d++) T.comb[m][d] = pre[m * 128 + d - 1];
+        // signing table: (j+1) 256^m B, j = 0..7
+        for (int j = 0; j < 8; j++) T.base[m][j] = T.comb[m][j + 1];
     }
 }
 
@@ -192,7 +213,37 @@ const ed25519_tables *device_tables(int hip_device) {
     return nullptr;
 }
 
+struct StreamScratch {
+    int dev;
+    hipStream_t stream;
+    Buf buf;
+};
+static std::mutex g_scratch_mu;
+static std::vector<StreamScratch *> g_scratch;
+
+void *stream_scratch(int hip_device, hipStream_t stream, size_t bytes) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    StreamScratch *e = nullptr;
+    for (StreamScratch *x : g_scratch)
+        if (x->dev == hip_device && x->stream == stream) e = x;
+    if (!e) {
+        e = new StreamScratch{hip_device, stream, Buf{}};
+        g_scratch.push_back(e);
+    }
+    if (e->buf.reserve(bytes) != hipSuccess) return nullptr;
+    return e->buf.ptr;
+}
+
 void pool_free_buffers() {
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        for (StreamScratch *x : g_scratch) {
+            if (hipSetDevice(x->dev) == hipSuccess) {
+                (void)hipDeviceSynchronize();
+                x->buf.release();
+            }
+        }
+    }
     std::lock_guard<std::mutex> g(g_pool_mu);
     if (g_num_devices <= 0) return;
     for (int d = 0; d < g_num_devices; d++) {

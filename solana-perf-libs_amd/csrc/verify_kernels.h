@@ -1,0 +1,55 @@
+// Verify kernel family and its launch configurations.
+#pragma once
+#include "runtime.h"
+
+namespace spl {
+
+// One launch configuration of the verify path; the autotuner picks among them.
+struct VerifyConfig {
+    uint8_t persistent;   // 0: one lane per signature, grid covers the batch;
+                          // 1: fixed grid of waves pulling 64-signature chunks
+                          //    from a device-side work queue
+    uint8_t window;       // variable-base window width W (4 or 5)
+    uint8_t waves_per_eu; // register-allocation target (1: up to 512 VGPRs per
+                          // lane, 2: up to 256)
+    uint8_t sort;         // persistent only: process signatures grouped by
+                          // SHA-512 block count, longest first
+    uint16_t waves_per_cu;// persistent only: resident waves per CU in the grid
+};
+
+constexpr VerifyConfig kDefaultVerifyConfig = {0, 4, 1, 0, 0};
+
+// Device-side description of one verify batch.  All pointers are device
+// pointers; n lives in device memory so a captured graph can be replayed for
+// any batch size up to the capacity it was built for.
+struct VerifyJob {
+    const uint8_t *packets;
+    uint64_t packets_size;
+    const uint32_t *lens, *pk_off, *sig_off, *msg_off;
+    uint8_t *out;
+    ge_cached *scratch;           // table storage (see verify_scratch_bytes)
+    const ed25519_tables *T;
+    const uint32_t *n_dev;        // number of signatures
+    uint32_t *work;               // [0]: queue head, [1..16]: bucket counts,
+                                  // [17..32]: bucket cursors
+    uint32_t *order;              // sort: n entries
+};
+
+// Bytes of scratch a launch of `cfg` needs for n signatures on this device.
+size_t verify_scratch_bytes(const VerifyConfig &cfg, uint32_t n, int num_cus);
+
+// Enqueues one verify of job on stream s.  The signature count is *job.n_dev
+// when job.n_dev is set (required for persistent configs: write it with
+// launch_set_n first), otherwise n_grid; plain configs launch a grid covering
+// n_grid signatures.  Persistent configs need job.work (33 words) and, with
+// sorting, job.order (n words).  Persistent configs run with waves_per_eu 1.
+hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint32_t n_grid,
+                             int num_cus, hipStream_t s);
+
+// Writes n to *n_dev in stream order.
+void launch_set_n(uint32_t *n_dev, uint32_t n, hipStream_t s);
+
+// Configurations the autotuner tries.
+int verify_config_candidates(VerifyConfig *out, int max);
+
+}  // namespace spl

@@ -132,8 +132,8 @@ class Event:
         self.handle = e.value
 
     def record(self, stream=None):
-        check(_hip.hipEventRecord(self.handle, stream.handle if stream else None),
-              "hipEventRecord")
+        """stream: a Stream, a raw hipStream_t handle, or None (null stream)."""
+        check(_hip.hipEventRecord(self.handle, _stream_handle(stream)), "hipEventRecord")
 
     def synchronize(self):
         check(_hip.hipEventSynchronize(self.handle), "hipEventSynchronize")
@@ -143,6 +143,32 @@ class Event:
         check(_hip.hipEventElapsedTime(ctypes.byref(ms), self.handle, end.handle),
               "hipEventElapsedTime")
         return ms.value
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    return stream.handle if hasattr(stream, "handle") else stream
+
+
+def upload_to(dptr, a):
+    """Synchronous copy of a host array to a raw device address."""
+    a = np.ascontiguousarray(a)
+    if a.nbytes:
+        check(_hip.hipMemcpy(dptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
+
+
+def download_from(dptr, count, dtype=np.uint8):
+    """Synchronous copy of `count` elements from a raw device address."""
+    out = np.empty(count, np.dtype(dtype))
+    if out.nbytes:
+        check(_hip.hipMemcpy(out.ctypes.data, dptr, out.nbytes, D2H), "hipMemcpy D2H")
+    return out
+
+
+def copy_d2d(dst, src, nbytes, stream=None):
+    check(_hip.hipMemcpyAsync(dst.ptr, src.ptr, nbytes, D2D, stream.handle if stream else None),
+          "hipMemcpyAsync D2D")
 
 
 def device_attribute(attr, device=0):

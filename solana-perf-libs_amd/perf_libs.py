@@ -32,6 +32,9 @@ EXPORTS = (
     "ed25519_get_checked_scalar", "ed25519_check_packed_ge_small_order",
     "ed25519_free_gpu_mem", "poh_verify_many_set_verbose", "poh_verify_many",
     "ed25519_verify_many_device", "ed25519_sign_many_device", "poh_verify_many_device",
+    "spl_verify_engine_create", "spl_verify_engine_destroy", "spl_verify_engine_buffers",
+    "spl_verify_engine_run", "spl_verify_engine_sync", "spl_verify_engine_autotune",
+    "spl_verify_engine_config",
 )
 
 
@@ -79,6 +82,21 @@ class PerfLibs:
         lib.ed25519_sign_many_device.restype = ctypes.c_int
         lib.poh_verify_many_device.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
         lib.poh_verify_many_device.restype = ctypes.c_int
+        lib.spl_verify_engine_create.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
+        lib.spl_verify_engine_create.restype = _vp
+        lib.spl_verify_engine_destroy.argtypes = [_vp]
+        lib.spl_verify_engine_destroy.restype = None
+        lib.spl_verify_engine_buffers.argtypes = [_vp] + [ctypes.POINTER(_vp)] * 7
+        lib.spl_verify_engine_buffers.restype = ctypes.c_int
+        lib.spl_verify_engine_run.argtypes = [_vp, ctypes.c_uint32]
+        lib.spl_verify_engine_run.restype = ctypes.c_int
+        lib.spl_verify_engine_sync.argtypes = [_vp]
+        lib.spl_verify_engine_sync.restype = ctypes.c_int
+        lib.spl_verify_engine_autotune.argtypes = [_vp, ctypes.c_uint32, ctypes.c_char_p,
+                                                   ctypes.c_size_t]
+        lib.spl_verify_engine_autotune.restype = ctypes.c_int
+        lib.spl_verify_engine_config.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t]
+        lib.spl_verify_engine_config.restype = ctypes.c_int
 
     # ------------------------------------------------------------ host API
     def init(self):
@@ -171,6 +189,51 @@ class PerfLibs:
 
     def poh_verify_many_device(self, d_hashes, d_counts, n, stream=0):
         return self.lib.poh_verify_many_device(d_hashes, d_counts, n, stream)
+
+
+class VerifyEngine:
+    """spl_verify_engine: device buffers + graph-captured tuned verify launch."""
+
+    def __init__(self, lib, max_signatures, max_packet_bytes):
+        self.lib = lib.lib
+        self.h = self.lib.spl_verify_engine_create(max_signatures, max_packet_bytes)
+        if not self.h:
+            raise RuntimeError("spl_verify_engine_create failed")
+        ptrs = [_vp() for _ in range(7)]
+        rc = self.lib.spl_verify_engine_buffers(self.h, *[ctypes.byref(p) for p in ptrs])
+        assert rc == 0
+        (self.d_packets, self.d_lens, self.d_pk, self.d_sig, self.d_msg, self.d_out,
+         self.stream) = [p.value for p in ptrs]
+        self.cap = max_signatures
+
+    def run(self, n):
+        return self.lib.spl_verify_engine_run(self.h, n)
+
+    def sync(self):
+        return self.lib.spl_verify_engine_sync(self.h)
+
+    def autotune(self, n):
+        buf = ctypes.create_string_buffer(8192)
+        rc = self.lib.spl_verify_engine_autotune(self.h, n, buf, len(buf))
+        if rc != 0:
+            raise RuntimeError(f"autotune failed: {rc}")
+        return buf.value.decode()
+
+    def config(self):
+        buf = ctypes.create_string_buffer(256)
+        self.lib.spl_verify_engine_config(self.h, buf, len(buf))
+        return buf.value.decode()
+
+    def close(self):
+        if self.h:
+            self.lib.spl_verify_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _LIB = None
