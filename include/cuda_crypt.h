@@ -154,6 +154,49 @@ int spl_verify_engine_autotune(spl_verify_engine *engine, uint32_t num_signature
                                size_t report_len);
 int spl_verify_engine_config(spl_verify_engine *engine, char *buf, size_t len);
 
+/* ------------------------------------------------------------------------
+ * Extension: serving loop (no counterpart in the reference, whose callers
+ * make one synchronous ed25519_verify_many / poh_verify_many call per batch).
+ * A server owns a device ring of hbm_budget_bytes (0: 1 GiB, at most 4 GiB)
+ * with a pinned host mirror, verify_streams + poh_streams streams (0: 6 and 2;
+ * at most 16 together), and one loop thread.  Requests are submitted from any
+ * thread and return a ticket (> 0) at once; the loop admits pending requests
+ * in arrival order into ring space as it frees up and, whenever a stream is
+ * idle, launches every admitted request of that kind as one batch (up to
+ * max_batch_signatures signatures or chains; 0: 65,536), so batches on
+ * different streams overlap on the device.
+ *
+ * spl_server_submit_verify takes one request in the ed25519_verify_many
+ * layout, with offsets relative to `packets` (packets_size bytes); a signature
+ * whose offsets fall outside its own packets is rejected (0).  `out` receives
+ * num_signatures mask bytes.  spl_server_submit_poh hashes `hashes` in place
+ * like poh_verify_many.  Every pointer passed must stay valid until
+ * spl_server_wait reports the ticket complete.  Tickets < 0: -2 bad arguments
+ * or a request larger than the ring or max_batch_signatures.
+ *
+ * spl_server_wait blocks up to timeout_ms (< 0: no limit) and returns 0 when
+ * the results are written, 1 on time-out (the ticket stays valid), 2 for an
+ * unknown ticket, 3 when a device error failed the batch (verify masks are
+ * written 0, PoH hashes are left unchanged).  On 0 and 3 the ticket is
+ * retired and times_ns (if not NULL) receives submit, admit and completion
+ * times from CLOCK_MONOTONIC.  spl_server_stats writes a JSON object with
+ * per-kind request counts and p50/p99/max latency (submit to completion) and
+ * optionally resets them.  spl_server_destroy finishes every submitted
+ * request first. */
+typedef struct spl_server spl_server;
+spl_server *spl_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch_signatures,
+                              uint32_t verify_streams, uint32_t poh_streams);
+void spl_server_destroy(spl_server *server);
+int64_t spl_server_submit_verify(spl_server *server, const uint8_t *packets, uint64_t packets_size,
+                                 const uint32_t *message_lens, const uint32_t *public_key_offsets,
+                                 const uint32_t *signature_offsets,
+                                 const uint32_t *message_start_offsets, uint32_t num_signatures,
+                                 uint8_t *out);
+int64_t spl_server_submit_poh(spl_server *server, uint8_t *hashes, const uint64_t *num_hashes_arr,
+                              size_t num_elems);
+int spl_server_wait(spl_server *server, int64_t ticket, int timeout_ms, uint64_t *times_ns);
+int spl_server_stats(spl_server *server, char *buf, size_t len, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ sign_kernel(const uint8_t *__restrict__ packets, uint64_t packets_size,
             const uint32_t *__restrict__ lens, const uint32_t *__restrict__ pk_off,
             const uint32_t *__restrict__ sk_off, const uint32_t *__restrict__ msg_off, uint32_t n,
             uint8_t *__restrict__ sigs, const ed25519_tables *__restrict__ T) {
+    raise_wave_priority();
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t *o = reinterpret_cast<uint32_t *>(sigs + 64ull * i);

@@ -27,7 +27,7 @@ static TuneEntry g_tune[kMaxDevices][33];
 
 static int size_class(uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; }
 
-static VerifyConfig tuned_plain_config(int dev, uint32_t n) {
+VerifyConfig tuned_plain_config(int dev, uint32_t n) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     if (dev >= 0 && dev < kMaxDevices) {
         const TuneEntry &e = g_tune[dev][size_class(n)];
@@ -36,7 +36,7 @@ static VerifyConfig tuned_plain_config(int dev, uint32_t n) {
     return kDefaultVerifyConfig;
 }
 
-static int device_cus(int dev) {
+int device_cus(int dev) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)

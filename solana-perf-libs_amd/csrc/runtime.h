@@ -40,6 +40,14 @@ extern bool g_verbose;
         }                                                                       \
     } while (0)
 
+// Short-lived kernels (verify, sign, copies) raise their waves' priority above
+// the default that PoH chain waves keep.  Among the waves sharing a SIMD the
+// instruction arbiter serves higher priority first and then the older wave; a
+// PoH wave always has a VALU instruction ready, so without this a verify wave
+// placed on the same SIMD as a running chain waits until that chain finishes
+// (measured: 1.1 ms verify launches stretched to 22-32 ms beside 33 ms chains).
+__device__ __forceinline__ void raise_wave_priority() { __builtin_amdgcn_s_setprio(1); }
+
 constexpr int kMaxDevices = 8;
 constexpr int kQueuesPerDevice = 8;
 
@@ -92,6 +100,11 @@ const ed25519_tables *device_tables(int hip_device);
 // it in stream order and launches on different streams never share it.
 // Returns nullptr on allocation failure.
 void *stream_scratch(int hip_device, hipStream_t stream, size_t bytes);
+
+// Enqueues SHA-256 chains: hashes[8*i..8*i+8) <- SHA-256^counts[i] of itself,
+// in place, for i < n (poh_verify.hip).
+hipError_t launch_poh(uint32_t *d_hashes, const uint64_t *d_counts, uint64_t n,
+                      hipStream_t stream);
 
 // Frees every cached device/pinned buffer (ed25519_free_gpu_mem).
 void pool_free_buffers();

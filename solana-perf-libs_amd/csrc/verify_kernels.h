@@ -52,4 +52,16 @@ void launch_set_n(uint32_t *n_dev, uint32_t n, hipStream_t s);
 // Configurations the autotuner tries.
 int verify_config_candidates(VerifyConfig *out, int max);
 
+// The fastest one-shot configuration the autotuner recorded for n's size
+// class on HIP device dev (kDefaultVerifyConfig when none was recorded).
+VerifyConfig tuned_plain_config(int dev, uint32_t n);
+
+// Compute units of HIP device dev (256 if the query fails).
+int device_cus(int dev);
+
+// Builds a VerifyJob for a one-shot launch (n_dev, work and order unset).
+VerifyJob make_verify_job(const uint8_t *packets, uint64_t size, const uint32_t *lens,
+                          const uint32_t *pk, const uint32_t *sig, const uint32_t *msg,
+                          uint8_t *out, void *scratch, const ed25519_tables *T);
+
 }  // namespace spl

@@ -40,6 +40,7 @@ __device__ __forceinline__ void verify_sig(const VerifyJob &job, uint32_t i, ge_
 template <int W, int WPE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 verify_plain_kernel(VerifyJob job, uint32_t n_arg) {
+    raise_wave_priority();
     const uint32_t n = job.n_dev ? *job.n_dev : n_arg;
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
     if (i >= n) return;
@@ -49,6 +50,7 @@ verify_plain_kernel(VerifyJob job, uint32_t n_arg) {
 template <int W, int WPE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 verify_persistent_kernel(VerifyJob job) {
+    raise_wave_priority();
     const uint32_t n = *job.n_dev;
     const uint32_t lane = threadIdx.x;
     ge_cached *Ai = job.scratch + ((size_t)blockIdx.x * 64u + lane) * VarWindow<W>::kTable;

@@ -34,7 +34,8 @@ EXPORTS = (
     "ed25519_verify_many_device", "ed25519_sign_many_device", "poh_verify_many_device",
     "spl_verify_engine_create", "spl_verify_engine_destroy", "spl_verify_engine_buffers",
     "spl_verify_engine_run", "spl_verify_engine_sync", "spl_verify_engine_autotune",
-    "spl_verify_engine_config",
+    "spl_verify_engine_config", "spl_server_create", "spl_server_destroy",
+    "spl_server_submit_verify", "spl_server_submit_poh", "spl_server_wait", "spl_server_stats",
 )
 
 
@@ -97,6 +98,20 @@ class PerfLibs:
         lib.spl_verify_engine_autotune.restype = ctypes.c_int
         lib.spl_verify_engine_config.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t]
         lib.spl_verify_engine_config.restype = ctypes.c_int
+        lib.spl_server_create.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint32]
+        lib.spl_server_create.restype = _vp
+        lib.spl_server_destroy.argtypes = [_vp]
+        lib.spl_server_destroy.restype = None
+        lib.spl_server_submit_verify.argtypes = [_vp, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp,
+                                                 ctypes.c_uint32, _vp]
+        lib.spl_server_submit_verify.restype = ctypes.c_int64
+        lib.spl_server_submit_poh.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        lib.spl_server_submit_poh.restype = ctypes.c_int64
+        lib.spl_server_wait.argtypes = [_vp, ctypes.c_int64, ctypes.c_int, _u64p]
+        lib.spl_server_wait.restype = ctypes.c_int
+        lib.spl_server_stats.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int]
+        lib.spl_server_stats.restype = ctypes.c_int
 
     # ------------------------------------------------------------ host API
     def init(self):
@@ -228,6 +243,75 @@ class VerifyEngine:
         if self.h:
             self.lib.spl_verify_engine_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """spl_server: continuous-batching request loop for verify and PoH requests.
+
+    The numpy arrays of a request are referenced here until its wait()
+    returns, as the C ABI requires."""
+
+    def __init__(self, lib, hbm_budget_bytes=0, max_batch_signatures=0, verify_streams=0,
+                 poh_streams=0):
+        self.lib = lib.lib
+        self.h = self.lib.spl_server_create(hbm_budget_bytes, max_batch_signatures,
+                                            verify_streams, poh_streams)
+        if not self.h:
+            raise RuntimeError("spl_server_create failed (no GPU, or bad arguments)")
+        self._live = {}
+        self._times = (ctypes.c_uint64 * 3)()
+
+    def submit_verify(self, packets, lens, pk_off, sig_off, msg_off, out=None):
+        """One request in the ed25519_verify_many layout (offsets relative to
+        `packets`).  Returns (ticket, out); out is filled on completion."""
+        tabs = [np.ascontiguousarray(t, dtype=np.uint32) for t in (lens, pk_off, sig_off, msg_off)]
+        packets = np.ascontiguousarray(packets, dtype=np.uint8)
+        n = tabs[0].size
+        if out is None:
+            out = np.zeros(max(n, 1), np.uint8)
+        t = self.lib.spl_server_submit_verify(self.h, packets.ctypes.data, packets.size,
+                                              *[x.ctypes.data for x in tabs], n, out.ctypes.data)
+        if t > 0:
+            self._live[t] = (packets, tabs, out)
+        return t, out
+
+    def submit_poh(self, hashes, counts):
+        """hashes: uint8 [32*n], hashed in place on completion."""
+        counts = np.ascontiguousarray(counts, dtype=np.uint64)
+        assert hashes.dtype == np.uint8 and hashes.flags.c_contiguous
+        assert hashes.size == 32 * counts.size
+        t = self.lib.spl_server_submit_poh(self.h, hashes.ctypes.data, counts.ctypes.data,
+                                           counts.size)
+        if t > 0:
+            self._live[t] = (hashes, counts)
+        return t
+
+    def wait(self, ticket, timeout_ms=-1):
+        """Returns (rc, (t_submit, t_admit, t_done) in CLOCK_MONOTONIC ns)."""
+        rc = self.lib.spl_server_wait(self.h, ticket, timeout_ms, self._times)
+        times = tuple(int(x) for x in self._times)
+        if rc in (0, 3):
+            self._live.pop(ticket, None)
+        return rc, times
+
+    def stats(self, reset=False):
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        rc = self.lib.spl_server_stats(self.h, buf, len(buf), int(reset))
+        assert rc == 0
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if self.h:
+            self.lib.spl_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
 
     def __del__(self):
         try:

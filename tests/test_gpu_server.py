@@ -1,0 +1,190 @@
+"""Serving loop (spl_server): every request's result is bit-identical whether
+it runs alone or batched with others, and equals the oracle's.
+
+Requests are cut from a validator-shaped pool (1-4 signatures per packet,
+5 % flipped, 1 % S+L, 1 % small-order keys) and a PoH chain pool; the expected
+verdicts and end states come from the oracle (the reference algorithm in C).
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import loadgen
+import perf_libs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pools(gpu_lib, oracle):
+    def expected(b, poh_in, counts):
+        mask = oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off, threads=8)
+        out = poh_in.copy()
+        oracle.poh_verify_many(out, counts, threads=8)
+        return mask, out
+
+    p = loadgen.build_pools(gpu_lib, seed=21, num_batches=16, poh_chains=96, poh_hashes=700,
+                            expected=expected)
+    print(f"pools: {p.b.n} signatures in {p.num_packets} packets, {p.num_chains} chains",
+          flush=True)
+    return p
+
+
+def _requests(pools, seed, n_verify, n_poh, max_packets=256, max_chains=48):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_verify):
+        npk = loadgen.log_uniform_int(rng, 1, max_packets)
+        p0 = int(rng.integers(0, pools.num_packets - npk + 1))
+        reqs.append(("verify", pools.verify_request(p0, npk)))
+    for _ in range(n_poh):
+        nch = loadgen.log_uniform_int(rng, 1, max_chains)
+        c0 = int(rng.integers(0, pools.num_chains - nch + 1))
+        reqs.append(("poh", pools.poh_request(c0, nch)))
+    return reqs
+
+
+def _submit(server, kind, req):
+    if kind == "verify":
+        packets, lens, pk, sig, msg, _ = req
+        t, out = server.submit_verify(packets, lens, pk, sig, msg)
+        return t, out
+    hashes, counts, _ = req
+    h = hashes.copy()
+    return server.submit_poh(h, counts), h
+
+
+def _result(kind, req, res):
+    return res[:req[5].size].copy() if kind == "verify" else res.copy()
+
+
+def _expected(kind, req):
+    return req[5] if kind == "verify" else req[2]
+
+
+def test_alone_equals_mix(gpu_lib, pools):
+    reqs = _requests(pools, seed=1, n_verify=48, n_poh=10)
+    server = perf_libs.Server(gpu_lib, hbm_budget_bytes=64 << 20)
+    try:
+        alone = []
+        for kind, req in reqs:
+            t, res = _submit(server, kind, req)
+            assert t > 0
+            rc, _ = server.wait(t)
+            assert rc == 0
+            alone.append(_result(kind, req, res))
+        st_alone = server.stats(reset=True)
+        assert st_alone["verify"]["batches"] == 48 and st_alone["poh"]["batches"] == 10
+
+        # the mix: verify and PoH requests submitted concurrently from two threads
+        subs = [None] * len(reqs)
+
+        def feed(kinds):
+            for i, (kind, req) in enumerate(reqs):
+                if kind in kinds:
+                    subs[i] = _submit(server, kind, req)
+
+        th = [threading.Thread(target=feed, args=({"verify"},)),
+              threading.Thread(target=feed, args=({"poh"},))]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        mix = []
+        for (kind, req), (t, res) in zip(reqs, subs):
+            assert t > 0
+            rc, times = server.wait(t)
+            assert rc == 0 and times[0] <= times[1] <= times[2]
+            mix.append(_result(kind, req, res))
+        st = server.stats()
+        print("mix stats:", st, flush=True)
+        # requests were batched together
+        assert st["verify"]["requests"] == 48 and st["verify"]["batches"] < 48
+        assert st["poh"]["requests"] == 10
+    finally:
+        server.close()
+    for (kind, req), a, m in zip(reqs, alone, mix):
+        exp = _expected(kind, req)
+        assert np.array_equal(a, exp), kind
+        assert np.array_equal(m, exp), kind
+    # and the plain drop-in call agrees on every verify request
+    for kind, req in reqs[:12]:
+        if kind == "verify":
+            packets, lens, pk, sig, msg, exp = req
+            assert np.array_equal(gpu_lib.verify_many([packets], 1232, lens, pk, sig, msg), exp)
+
+
+def test_small_ring_backpressure(gpu_lib, pools):
+    """A ring that holds only a few requests: admission waits for space, the
+    ring wraps, results stay exact."""
+    reqs = _requests(pools, seed=2, n_verify=60, n_poh=6, max_packets=200)
+    budget = 1 << 20
+    server = perf_libs.Server(gpu_lib, hbm_budget_bytes=budget, max_batch_signatures=4096,
+                              verify_streams=2, poh_streams=1)
+    try:
+        subs = [_submit(server, kind, req) for kind, req in reqs]
+        for (kind, req), (t, res) in zip(reqs, subs):
+            assert t > 0
+            assert server.wait(t)[0] == 0
+            assert np.array_equal(_result(kind, req, res), _expected(kind, req)), kind
+        st = server.stats()
+        assert st["ring_peak_bytes"] <= budget
+        # a request larger than the whole ring is refused up front
+        big = np.zeros(2 << 20, np.uint8)
+        t, _ = server.submit_verify(big, [10], [0], [64], [96])
+        assert t == -2
+    finally:
+        server.close()
+
+
+def test_edges(gpu_lib, pools):
+    server = perf_libs.Server(gpu_lib, hbm_budget_bytes=8 << 20)
+    try:
+        # empty requests complete at once
+        t, _ = server.submit_verify(np.zeros(16, np.uint8), [], [], [], [])
+        assert t > 0 and server.wait(t, 0)[0] == 0
+        t = server.submit_poh(np.zeros(0, np.uint8), np.zeros(0, np.uint64))
+        assert t > 0 and server.wait(t, 0)[0] == 0
+        assert server.wait(987654321)[0] == 2
+        # offsets outside the request's own packets reject that signature only
+        packets, lens, pk, sig, msg, exp = pools.verify_request(3, 5)
+        lens2, msg2 = lens.copy(), msg.copy()
+        lens2[0] = packets.size - msg2[0] + 1          # message runs one byte past the end
+        t, out = server.submit_verify(packets, lens2, pk, sig, msg2)
+        assert server.wait(t)[0] == 0
+        want = exp.copy()
+        want[0] = 0
+        assert np.array_equal(out[:want.size], want)
+        # zero hash counts leave the chain unchanged
+        h = np.arange(64, dtype=np.uint8)
+        t = server.submit_poh(h, np.array([0, 3], np.uint64))
+        assert server.wait(t)[0] == 0
+        assert np.array_equal(h[:32], np.arange(32, dtype=np.uint8))
+        assert np.array_equal(h[32:], pools_oracle_chain(np.arange(32, 64, dtype=np.uint8), 3))
+    finally:
+        server.close()
+
+
+def pools_oracle_chain(h, n):
+    import hashlib
+    b = h.tobytes()
+    for _ in range(n):
+        b = hashlib.sha256(b).digest()
+    return np.frombuffer(b, np.uint8)
+
+
+def test_poisson_trace(gpu_lib, pools):
+    """A short open-loop Poisson run through the load generator."""
+    rng = np.random.default_rng(9)
+    trace = loadgen.make_trace(pools, rng, sig_rate=300_000, duration_s=0.4, poh_rate=20,
+                               max_packets=512, max_chains=48)
+    server = perf_libs.Server(gpu_lib, hbm_budget_bytes=256 << 20)
+    try:
+        recs, span, _ = loadgen.run_served(server, trace)
+    finally:
+        server.close()
+    s = loadgen.summarize(recs, span)
+    print("poisson:", s, flush=True)
+    assert s["verify"]["all_exact"] and s.get("poh", {"all_exact": True})["all_exact"]
+    assert s["verify"]["requests"] > 10

@@ -104,7 +104,8 @@ def _run_without_gpu(code):
 def test_no_gpu_behaviour():
     """Without a device: init is false, poh_verify_many returns 1 and leaves the
     hashes alone, verify_many rejects everything instead of crashing
-    (the reference dereferences NULL there, verify.cu:265-267)."""
+    (the reference dereferences NULL there, verify.cu:265-267), and the engine
+    and server constructors refuse."""
     code = r"""
 import sys, numpy as np
 sys.path.insert(0, 'solana-perf-libs_amd')
@@ -120,6 +121,13 @@ pkt[:64] = np.frombuffer(L.sign(b'x' * 100, pk, sk), np.uint8)
 pkt[64:96] = np.frombuffer(pk, np.uint8); pkt[96:196] = ord('x')
 out = L.verify_many([pkt], 256, [100], [64], [0], [96])
 assert out.tolist() == [0]
+assert L.lib.spl_server_create(0, 0, 0, 0) is None
+assert L.lib.spl_verify_engine_create(16, 4096) is None
+try:
+    perf_libs.Server(L)
+    raise SystemExit('Server() must raise without a GPU')
+except RuntimeError:
+    pass
 print('ok')
 """
     r = _run_without_gpu(code)
