@@ -137,8 +137,9 @@ int poh_verify_many_device(uint8_t *d_hashes, const uint64_t *d_num_hashes, size
  * spl_verify_engine_buffers (same layout as ed25519_verify_many_device),
  * calls spl_verify_engine_run, then spl_verify_engine_sync and reads the mask.
  * spl_verify_engine_autotune times every launch configuration (one-shot or
- * persistent, window width 4/5, register target, length bucketing, waves per
- * CU) on the batch currently in the buffers, keeps the fastest and records it
+ * persistent, window width 4/5, full 253-bit or half-size ~128-bit scalars,
+ * register target, length bucketing, waves per CU) on the batch currently in
+ * the buffers, keeps the fastest and records it
  * for that batch-size class on the device, where the ABI calls pick it up.
  * An engine is used by one thread at a time.  Functions returning int: 0 ok,
  * 2 bad arguments, 3 device error. */
@@ -153,6 +154,12 @@ int spl_verify_engine_sync(spl_verify_engine *engine);
 int spl_verify_engine_autotune(spl_verify_engine *engine, uint32_t num_signatures, char *report,
                                size_t report_len);
 int spl_verify_engine_config(spl_verify_engine *engine, char *buf, size_t len);
+/* The autotuner's candidate configurations: returns their number and writes
+ * their names, one per line, to buf (if not NULL).  spl_verify_engine_use
+ * makes candidate `index` the engine's configuration (recapturing its graph)
+ * without timing anything. */
+int spl_verify_engine_candidates(spl_verify_engine *engine, char *buf, size_t len);
+int spl_verify_engine_use(spl_verify_engine *engine, int index);
 
 /* ------------------------------------------------------------------------
  * Extension: serving loop (no counterpart in the reference, whose callers

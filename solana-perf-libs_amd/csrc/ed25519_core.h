@@ -28,6 +28,7 @@
 // for it.  The group law is exact, so the result is the same point either way.
 #pragma once
 #include "ge25519.h"
+#include "lattice.h"
 #include "sc25519.h"
 #include "sha2.h"
 
@@ -258,6 +259,163 @@ SPL_HD int verify_one(const uint32_t sig[16], const uint32_t pk[8], const MsgRea
     if (!verify_prepare(sig, pk, msg, T, h, A)) return 0;
     build_multiples(Ai, VarWindow<W>::kTable, A, T.k.d2);
     return verify_finish<W>(sig, h, Ai, T);
+}
+
+// ------------------------------------------------------------ verify, half-size scalars
+
+// y (the low 255 bits of an encoding) < p = 2^255 - 19
+SPL_HD uint32_t enc_y_canonical(const uint32_t enc[8]) {
+    uint32_t mid = 0xffffffffu;
+#pragma unroll
+    for (int i = 1; i < 7; i++) mid &= enc[i];
+    const uint32_t ge_p = ((enc[7] & 0x7fffffffu) == 0x7fffffffu) & (mid == 0xffffffffu) &
+                          (enc[0] >= 0xffffffedu);
+    return ge_p ^ 1u;
+}
+
+// a <<= n for 0 <= n < 256 (bits shifted out are dropped)
+SPL_HD void shl256(uint32_t a[8], int n) {
+    const int q = n >> 5, r = n & 31;
+    uint32_t w[8];
+#pragma unroll
+    for (int i = 7; i >= 0; i--) {
+        const uint32_t lo = i > 0 ? a[i - 1] : 0u;
+        w[i] = r ? (a[i] << r) | (lo >> (32 - r)) : a[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) v = (j + q == i) ? w[j] : v;
+        a[i] = v;
+    }
+}
+
+// Stage 1 of the half-size verify (see lattice.h): the same acceptance checks
+// as verify_prepare, plus -R (decoded: the equation below needs the point), the
+// half scalars x, y of h and sp = y S mod L.  P1 receives -A, or A when x < 0,
+// so the multiplication below always uses |x|.  Returns 0 when rejected.
+//   accept <=> [sp]B + [|x|]P1 + [y](-R) = O
+// R's bytes must be the canonical encoding of a decodable point that is not of
+// small order: encode() only produces canonical encodings, so the reference's
+// byte comparison rejects every other R.
+SPL_HD int verify_prepare_half(const uint32_t sig[16], const uint32_t pk[8], const MsgReader &msg,
+                               const ed25519_tables &T, uint32_t x[8], uint32_t y[8],
+                               uint32_t sp[8], ge_p3 &P1, ge_p3 &R) {
+    if (sig[15] & 0xf0000000u) return 0;           // S >= 2^252
+    if (!enc_y_canonical(sig)) return 0;           // R not canonical
+    if (y_low_order(sig)) return 0;                // R of order <= 4
+    uint32_t pre[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) { pre[i] = sig[i]; pre[8 + i] = pk[i]; }
+    uint64_t st[8];
+    sha512_prefixed(st, pre, 16, msg);
+    uint32_t hw[16], h[8];
+    sha512_digest_le_words(hw, st);
+    sc_reduce512(h, hw);
+    if (ge_decode_neg(P1, pk, T.k) != 0) return 0;
+    if (y_low_order(pk)) return 0;
+    if (ge_decode_neg(R, sig, T.k) != 0) return 0;
+    uint32_t x_neg;
+    half_scalars(h, x, x_neg, y);
+    const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    sc_muladd(sp, y, sig + 8, zero);
+    if (x_neg) {
+        fe_neg(P1.X, P1.X);
+        fe_neg(P1.T, P1.T);
+    }
+    return 1;
+}
+
+// Bits needed by the windowed multiplication of stage 2 for this lane.
+SPL_HD int half_bits(const uint32_t x[8], const uint32_t y[8]) {
+    uint32_t o[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = x[i] | y[i];
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        if (o[i]) r = 32 * i + bitlen32(o[i]);
+    return r;
+}
+
+// Windows of W bits covering `bits` (one extra bit for the signed digits).
+template <int W>
+SPL_HD int half_windows(int bits) { return (bits + W) / W; }
+
+// Stage 2: K windows of W bits (K W <= 256, the same K in every lane of a
+// wave); T1, Tr hold 0..2^(W-1) times P1 and -R.  Returns 1 when
+// [sp]B + [x]P1 + [y](-R) is the identity.
+template <int W>
+SPL_HD int verify_finish_half(const uint32_t x[8], const uint32_t y[8], const uint32_t sp[8],
+                              int K, const ge_cached *T1, const ge_cached *Tr,
+                              const ed25519_tables &T) {
+    uint32_t a[8], b[8], s[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) { a[i] = x[i]; b[i] = y[i]; s[i] = sp[i]; }
+    shl256(a, 256 - K * W);
+    shl256(b, 256 - K * W);
+    uint32_t sbelow = 0;
+    int m = 0;                        // comb digits of sp consumed
+    ge_p3 pb, u;
+    ge_p3_0(pb);
+    ge_p1p1 t, tb;
+    ge_p2 r;
+    ge_p2_0(r);
+    for (int j = K - 1; j >= 0; j--) {
+        const int d1 = take_top_digit<W>(a);
+        const int d2 = take_top_digit<W>(b);
+        if (m < 32) {
+            const int df = take_low_byte_digit(s, sbelow);
+            ge_add_precomp(tb, pb, T.comb[m][df < 0 ? -df : df], df < 0);
+            ge_p1p1_to_p3(pb, tb);
+            m++;
+        }
+        if (j != K - 1) {
+#pragma unroll
+            for (int k = 0; k < W - 1; k++) {
+                ge_p2_dbl(t, r);
+                ge_p1p1_to_p2(r, t);
+            }
+            ge_p2_dbl(t, r);
+            ge_p1p1_to_p3(u, t);
+        } else {
+            ge_p3_0(u);
+        }
+        ge_add_cached(t, u, T1[d1 < 0 ? -d1 : d1], d1 < 0);
+        ge_p1p1_to_p3(u, t);
+        ge_add_cached(t, u, Tr[d2 < 0 ? -d2 : d2], d2 < 0);
+        if (j != 0) ge_p1p1_to_p2(r, t);
+    }
+    for (; m < 32; m++) {             // comb digits left when K < 32
+        const int df = take_low_byte_digit(s, sbelow);
+        ge_add_precomp(tb, pb, T.comb[m][df < 0 ? -df : df], df < 0);
+        ge_p1p1_to_p3(pb, tb);
+    }
+    ge_cached cb;
+    ge_p1p1_to_p3(u, t);
+    ge_p3_to_cached(cb, pb, T.k.d2);
+    ge_add_cached(t, u, cb, 0);
+    ge_p1p1_to_p2(r, t);
+    fe dz;
+    fe_sub(dz, r.Y, r.Z);
+    return (int)(fe_iszero(r.X) & fe_iszero(dz));
+}
+
+// The half-size verify for one signature with its own window count (host
+// and per-lane use).  Tables: caller storage for 2 * VarWindow<W>::kTable
+// cached points.
+template <int W = 4>
+SPL_HD int verify_half_one(const uint32_t sig[16], const uint32_t pk[8], const MsgReader &msg,
+                           const ed25519_tables &T, ge_cached *tables) {
+    uint32_t x[8], y[8], sp[8];
+    ge_p3 P1, R;
+    if (!verify_prepare_half(sig, pk, msg, T, x, y, sp, P1, R)) return 0;
+    constexpr int kT = VarWindow<W>::kTable;
+    build_multiples(tables, kT, P1, T.k.d2);
+    build_multiples(tables + kT, kT, R, T.k.d2);
+    return verify_finish_half<W>(x, y, sp, half_windows<W>(half_bits(x, y)), tables,
+                                 tables + kT, T);
 }
 
 // ------------------------------------------------------------ fixed-base mult

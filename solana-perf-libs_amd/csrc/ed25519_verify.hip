@@ -314,10 +314,34 @@ int spl_verify_engine_sync(spl_verify_engine *eng) {
 
 int spl_verify_engine_config(spl_verify_engine *eng, char *buf, size_t len) {
     if (!eng || !buf || !len) return 2;
-    snprintf(buf, len, "%s W=%u waves_per_eu=%u sort=%u waves_per_cu=%u",
-             eng->cfg.persistent ? "persistent" : "one-shot", eng->cfg.window,
-             eng->cfg.waves_per_eu, eng->cfg.sort, eng->cfg.waves_per_cu);
+    config_name(eng->cfg, buf, len);
     return 0;
+}
+
+int spl_verify_engine_candidates(spl_verify_engine *eng, char *buf, size_t len) {
+    if (!eng) return -2;
+    VerifyConfig cand[64];
+    const int nc = verify_config_candidates(cand, 64);
+    if (buf && len) {
+        std::string all;
+        for (int i = 0; i < nc; i++) {
+            char line[128];
+            config_name(cand[i], line, sizeof line);
+            all += line;
+            all += "\n";
+        }
+        snprintf(buf, len, "%s", all.c_str());
+    }
+    return nc;
+}
+
+int spl_verify_engine_use(spl_verify_engine *eng, int index) {
+    if (!eng) return 2;
+    VerifyConfig cand[64];
+    const int nc = verify_config_candidates(cand, 64);
+    if (index < 0 || index >= nc) return 2;
+    eng->cfg = cand[index];
+    return eng->capture() == hipSuccess ? 0 : 3;
 }
 
 // Times every candidate configuration on the batch currently in the engine's
@@ -326,8 +350,8 @@ int spl_verify_engine_config(spl_verify_engine *eng, char *buf, size_t len) {
 int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
                                size_t report_len) {
     if (!eng || n == 0 || n > eng->cap_sigs) return 2;
-    VerifyConfig cand[32];
-    const int nc = verify_config_candidates(cand, 32);
+    VerifyConfig cand[64];
+    const int nc = verify_config_candidates(cand, 64);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return 3;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -353,10 +377,9 @@ int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
                 rc = 3;
         }
         if (rc) break;
-        char line[160];
-        snprintf(line, sizeof line, "%s W=%u wpe=%u sort=%u wpc=%u: %.3f ms\n",
-                 c.persistent ? "persistent" : "one-shot", c.window, c.waves_per_eu, c.sort,
-                 c.waves_per_cu, ms);
+        char name[128], line[160];
+        config_name(c, name, sizeof name);
+        snprintf(line, sizeof line, "%s: %.3f ms\n", name, ms);
         rep += line;
         if (ms < best_ms) { best_ms = ms; best = c; }
         if (!c.persistent && ms < best_plain_ms) { best_plain_ms = ms; best_plain = c; }

@@ -15,9 +15,12 @@ struct VerifyConfig {
     uint8_t sort;         // persistent only: process signatures grouped by
                           // SHA-512 block count, longest first
     uint16_t waves_per_cu;// persistent only: resident waves per CU in the grid
+    uint8_t half;         // 0: [h](-A) + [S]B with a 253-bit h, compared by
+                          //    encoding; 1: half-size scalars (lattice.h),
+                          //    [y S]B + [x](+-A) + [y](-R) = O, about 128-bit x, y
 };
 
-constexpr VerifyConfig kDefaultVerifyConfig = {0, 4, 1, 0, 0};
+constexpr VerifyConfig kDefaultVerifyConfig = {0, 4, 1, 0, 0, 1};
 
 // Device-side description of one verify batch.  All pointers are device
 // pointers; n lives in device memory so a captured graph can be replayed for
@@ -51,6 +54,9 @@ void launch_set_n(uint32_t *n_dev, uint32_t n, hipStream_t s);
 
 // Configurations the autotuner tries.
 int verify_config_candidates(VerifyConfig *out, int max);
+
+// Human-readable name of a configuration.
+void config_name(const VerifyConfig &c, char *buf, size_t len);
 
 // The fastest one-shot configuration the autotuner recorded for n's size
 // class on HIP device dev (kDefaultVerifyConfig when none was recorded).

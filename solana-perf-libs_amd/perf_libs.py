@@ -34,7 +34,8 @@ EXPORTS = (
     "ed25519_verify_many_device", "ed25519_sign_many_device", "poh_verify_many_device",
     "spl_verify_engine_create", "spl_verify_engine_destroy", "spl_verify_engine_buffers",
     "spl_verify_engine_run", "spl_verify_engine_sync", "spl_verify_engine_autotune",
-    "spl_verify_engine_config", "spl_server_create", "spl_server_destroy",
+    "spl_verify_engine_config", "spl_verify_engine_candidates", "spl_verify_engine_use",
+    "spl_server_create", "spl_server_destroy",
     "spl_server_submit_verify", "spl_server_submit_poh", "spl_server_wait", "spl_server_stats",
 )
 
@@ -98,6 +99,10 @@ class PerfLibs:
         lib.spl_verify_engine_autotune.restype = ctypes.c_int
         lib.spl_verify_engine_config.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t]
         lib.spl_verify_engine_config.restype = ctypes.c_int
+        lib.spl_verify_engine_candidates.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t]
+        lib.spl_verify_engine_candidates.restype = ctypes.c_int
+        lib.spl_verify_engine_use.argtypes = [_vp, ctypes.c_int]
+        lib.spl_verify_engine_use.restype = ctypes.c_int
         lib.spl_server_create.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_uint32]
         lib.spl_server_create.restype = _vp
@@ -233,6 +238,18 @@ class VerifyEngine:
         if rc != 0:
             raise RuntimeError(f"autotune failed: {rc}")
         return buf.value.decode()
+
+    def candidates(self):
+        buf = ctypes.create_string_buffer(8192)
+        n = self.lib.spl_verify_engine_candidates(self.h, buf, len(buf))
+        names = [l for l in buf.value.decode().splitlines() if l]
+        assert len(names) == n
+        return names
+
+    def use(self, index):
+        rc = self.lib.spl_verify_engine_use(self.h, index)
+        if rc != 0:
+            raise RuntimeError(f"spl_verify_engine_use({index}) failed: {rc}")
 
     def config(self):
         buf = ctypes.create_string_buffer(256)

@@ -55,3 +55,61 @@ def test_engine_bounds(gpu_lib, oracle):
         assert eng.run(0) == 0
     finally:
         eng.close()
+
+
+def _torsion_batch(oracle, n=384, seed=5):
+    """Signatures under keys A + T and with R + T for each of the 8 points T of
+    order dividing 8: the reference accepts A + T exactly when [h]T = O, so
+    these rows separate an exact check from a cofactored one."""
+    import ed25519_math as M
+    rng = np.random.default_rng(seed)
+    tors = [M.decode_lenient(t) for t in W.TORSION_CANONICAL]
+    stride = 256
+    buf = np.zeros((n, stride), np.uint8)
+    for i in range(n):
+        pk, sk = oracle.create_keypair(rng.integers(0, 256, 32, dtype=np.uint8).tobytes())
+        T = tors[i % 8]
+        kind = (i // 8) % 3
+        key = M.encode(M.add(M.decode_lenient(pk), T)) if kind != 1 else pk
+        msg = rng.integers(0, 256, int(rng.integers(0, 120)), dtype=np.uint8).tobytes()
+        sig = oracle.sign(msg, key, sk)
+        if kind >= 1:
+            sig = M.encode(M.add(M.decode_lenient(sig[:32]), T)) + sig[32:]
+        buf[i, :64] = np.frombuffer(sig, np.uint8)
+        buf[i, 64:96] = np.frombuffer(key, np.uint8)
+        buf[i, 96:96 + len(msg)] = np.frombuffer(msg, np.uint8)
+        buf[i, 255] = len(msg)
+    base = np.arange(n, dtype=np.uint32) * stride
+    lens = buf[:, 255].astype(np.uint32)
+    flat = buf.reshape(-1)
+    return W.Batch(flat, stride, [flat], lens, base + 64, base.copy(), base + 96)
+
+
+def test_every_configuration(gpu_lib, oracle):
+    """Each candidate configuration (one-shot / persistent, W = 4 / 5, full or
+    half-size scalars, register targets, bucketing) gives the oracle's mask."""
+    import perf_libs
+    batches = [
+        W.verify_validator(oracle.sign, oracle.create_keypair, seed=23, num_batches=10,
+                           num_keys=64, flip_frac=0.1, noncanon_frac=0.03,
+                           small_order_frac=0.03),
+        _torsion_batch(oracle),
+    ]
+    wants = [oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off, threads=8)
+             for b in batches]
+    assert 0 < wants[1].sum() < wants[1].size
+    cap = max(b.n for b in batches)
+    eng = perf_libs.VerifyEngine(gpu_lib, cap, max(b.packets.size for b in batches))
+    try:
+        names = eng.candidates()
+        assert any("scalars=half" in nm for nm in names)
+        for k, name in enumerate(names):
+            eng.use(k)
+            for b, want in zip(batches, wants):
+                _load(eng, b)
+                assert eng.run(b.n) == 0 and eng.sync() == 0
+                got = _mask(eng, b.n)
+                assert np.array_equal(got, want), (name, b.n, np.nonzero(got != want)[0][:10])
+            print(name, "ok", flush=True)
+    finally:
+        eng.close()
