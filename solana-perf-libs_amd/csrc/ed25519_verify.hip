@@ -28,11 +28,19 @@ static TuneEntry g_tune[kMaxDevices][33];
 static int size_class(uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; }
 
 VerifyConfig tuned_plain_config(int dev, uint32_t n) {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    if (dev >= 0 && dev < kMaxDevices) {
-        const TuneEntry &e = g_tune[dev][size_class(n)];
-        if (e.valid) return e.best_plain;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        if (dev >= 0 && dev < kMaxDevices) {
+            const TuneEntry &e = g_tune[dev][size_class(n)];
+            if (e.valid) return e.best_plain;
+        }
     }
+    // Untuned: a launch that fits one wave per SIMD is bound by one lane's
+    // latency, which two lanes per signature shorten (measured 0.69-0.75 ms
+    // against 0.93-0.98 ms for 512-16384 signatures); beyond that the pair
+    // kernel's extra total work costs more than it saves.
+    const uint32_t simds = 4u * (uint32_t)device_cus(dev);
+    if (2ull * n <= 64ull * simds) return kPairVerifyConfig;
     return kDefaultVerifyConfig;
 }
 

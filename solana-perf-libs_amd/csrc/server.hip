@@ -570,7 +570,7 @@ spl_server *spl_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch_sign
         if (ok && s.kind == kVerify) {
             // sized for a full batch under the widest window, so the loop
             // never allocates in steady state
-            const VerifyConfig widest = {0, 5, 1, 0, 0, 1};
+            const VerifyConfig widest = {0, 5, 1, 0, 0, 1, 1};
             ok = sv->ensure_scratch(s, verify_scratch_bytes(widest, max_batch_signatures,
                                                             sv->cus)) == hipSuccess;
         }

@@ -18,9 +18,13 @@ struct VerifyConfig {
     uint8_t half;         // 0: [h](-A) + [S]B with a 253-bit h, compared by
                           //    encoding; 1: half-size scalars (lattice.h),
                           //    [y S]B + [x](+-A) + [y](-R) = O, about 128-bit x, y
+    uint8_t lanes;        // lanes per signature: 1, or 2 (half only, one-shot only):
+                          //    one lane runs the x(+-A) chain, its neighbour the
+                          //    y(-R) chain, each with half of the comb digits
 };
 
-constexpr VerifyConfig kDefaultVerifyConfig = {0, 4, 1, 0, 0, 1};
+constexpr VerifyConfig kDefaultVerifyConfig = {0, 4, 1, 0, 0, 1, 1};
+constexpr VerifyConfig kPairVerifyConfig = {0, 4, 1, 0, 0, 1, 2};
 
 // Device-side description of one verify batch.  All pointers are device
 // pointers; n lives in device memory so a captured graph can be replayed for
@@ -59,7 +63,9 @@ int verify_config_candidates(VerifyConfig *out, int max);
 void config_name(const VerifyConfig &c, char *buf, size_t len);
 
 // The fastest one-shot configuration the autotuner recorded for n's size
-// class on HIP device dev (kDefaultVerifyConfig when none was recorded).
+// class on HIP device dev; when none was recorded, two lanes per signature
+// (kPairVerifyConfig) if the launch fits one wave per SIMD, else
+// kDefaultVerifyConfig.
 VerifyConfig tuned_plain_config(int dev, uint32_t n);
 
 // Compute units of HIP device dev (256 if the query fails).

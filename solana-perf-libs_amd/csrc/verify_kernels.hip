@@ -78,6 +78,149 @@ verify_plain_kernel(VerifyJob job, uint32_t n_arg) {
     verify_sig<W, HALF>(job, i, job.scratch + (size_t)i * lane_tables<W, HALF>());
 }
 
+// The value held by lane ^ 1 (ds_swizzle, bit mode: and 0x1f, or 0, xor 1).
+// (HIP's generic __shfl_xor here sent the compiler into a >20-minute build.)
+__device__ __forceinline__ int32_t pair_swap(int32_t v) {
+    return __builtin_amdgcn_ds_swizzle(v, 0x041f);
+}
+
+// Two lanes per signature (half-size scalars).  Lane role 0 ("A lane")
+// multiplies |x| by +-A, role 1 ("R lane") y by -R; the comb digits of
+// sp = y S mod L are split between them (bytes 0..15 and 16..31: the signed
+// radix-256 recoding carries only the top bit of byte 15 across the split).
+// Each lane ends with its partial sum; the two are exchanged within the pair
+// and both lanes test their total for the identity.  Every exit is taken by
+// both lanes of a pair together, so the exchange never reads an inactive lane.
+template <int W>
+__device__ __forceinline__ void verify_sig_pair(const VerifyJob &job, uint32_t i, uint32_t role,
+                                                ge_cached *tab) {
+    const ed25519_tables &T = *job.T;
+    const uint64_t so = job.sig_off[i], po = job.pk_off[i], mo = job.msg_off[i],
+                   ml = job.lens[i];
+    if (so + 64 > job.packets_size || po + 32 > job.packets_size ||
+        mo + ml > job.packets_size) {
+        if (!role) job.out[i] = 0;
+        return;
+    }
+    uint32_t sig[16], pk[8];
+    load_words(sig, job.packets + so, 16);
+    load_words(pk, job.packets + po, 8);
+    MsgReader msg{job.packets + mo, (uint32_t)ml};
+    // the same checks as verify_prepare_half, evaluated by both lanes
+    if ((sig[15] & 0xf0000000u) || !enc_y_canonical(sig) || y_low_order(sig) ||
+        y_low_order(pk)) {
+        if (!role) job.out[i] = 0;
+        return;
+    }
+    uint32_t pre[16];
+#pragma unroll
+    for (int k = 0; k < 8; k++) { pre[k] = sig[k]; pre[8 + k] = pk[k]; }
+    uint64_t st[8];
+    sha512_prefixed(st, pre, 16, msg);
+    uint32_t hw[16], h[8];
+    sha512_digest_le_words(hw, st);
+    sc_reduce512(h, hw);
+    ge_p3 P;
+    uint32_t enc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) enc[k] = role ? sig[k] : pk[k];
+    int dec = ge_decode_neg(P, enc, T.k) == 0;      // -A or -R
+    dec &= pair_swap(dec);
+    if (!dec) {
+        if (!role) job.out[i] = 0;
+        return;
+    }
+    uint32_t x[8], y[8], sp[8], x_neg;
+    half_scalars(h, x, x_neg, y);
+    const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    sc_muladd(sp, y, sig + 8, zero);
+    uint32_t a[8], s[8], below = role ? sp[3] >> 31 : 0u;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        a[k] = role ? y[k] : x[k];
+        s[k] = role ? (k < 4 ? sp[k + 4] : 0u) : sp[k];
+    }
+    if (!role && x_neg) {
+        fe_neg(P.X, P.X);
+        fe_neg(P.T, P.T);
+    }
+    constexpr int kT = VarWindow<W>::kTable;
+    build_multiples(tab, kT, P, T.k.d2);
+    int bits = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (a[k]) bits = 32 * k + bitlen32(a[k]);
+    const int K = wave_max7(half_windows<W>(bits));
+    shl256(a, 256 - K * W);
+    const int mbase = role ? 16 : 0;
+    int m = 0;
+    ge_p3 pb, u;
+    ge_p3_0(pb);
+    ge_p1p1 t, tb;
+    ge_p2 r;
+    ge_p2_0(r);
+    for (int j = K - 1; j >= 0; j--) {
+        const int d = take_top_digit<W>(a);
+        if (m < 16) {
+            const int df = take_low_byte_digit(s, below);
+            ge_add_precomp(tb, pb, T.comb[mbase + m][df < 0 ? -df : df], df < 0);
+            ge_p1p1_to_p3(pb, tb);
+            m++;
+        }
+        if (j != K - 1) {
+#pragma unroll
+            for (int k = 0; k < W - 1; k++) {
+                ge_p2_dbl(t, r);
+                ge_p1p1_to_p2(r, t);
+            }
+            ge_p2_dbl(t, r);
+            ge_p1p1_to_p3(u, t);
+        } else {
+            ge_p3_0(u);
+        }
+        ge_add_cached(t, u, tab[d < 0 ? -d : d], d < 0);
+        if (j != 0) ge_p1p1_to_p2(r, t);
+    }
+    for (; m < 16; m++) {
+        const int df = take_low_byte_digit(s, below);
+        ge_add_precomp(tb, pb, T.comb[mbase + m][df < 0 ? -df : df], df < 0);
+        ge_p1p1_to_p3(pb, tb);
+    }
+    // this lane's part: chain + comb half
+    ge_cached c;
+    ge_p1p1_to_p3(u, t);
+    ge_p3_to_cached(c, pb, T.k.d2);
+    ge_add_cached(t, u, c, 0);
+    ge_p1p1_to_p3(u, t);
+    // add the partner's part
+    ge_p3 v;
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        v.X.v[k] = pair_swap(u.X.v[k]);
+        v.Y.v[k] = pair_swap(u.Y.v[k]);
+        v.Z.v[k] = pair_swap(u.Z.v[k]);
+        v.T.v[k] = pair_swap(u.T.v[k]);
+    }
+    ge_p3_to_cached(c, v, T.k.d2);
+    ge_add_cached(t, u, c, 0);
+    ge_p1p1_to_p2(r, t);
+    fe dz;
+    fe_sub(dz, r.Y, r.Z);
+    const uint32_t ok = fe_iszero(r.X) & fe_iszero(dz);
+    if (!role) job.out[i] = (uint8_t)ok;
+}
+
+template <int W, int WPE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+verify_pair_kernel(VerifyJob job, uint32_t n_arg) {
+    raise_wave_priority();
+    const uint32_t n = job.n_dev ? *job.n_dev : n_arg;
+    const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t i = g >> 1;
+    if (i >= n) return;                    // both lanes of the pair
+    verify_sig_pair<W>(job, i, g & 1u, job.scratch + (size_t)g * VarWindow<W>::kTable);
+}
+
 template <int W, int WPE, int HALF>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 verify_persistent_kernel(VerifyJob job) {
@@ -161,6 +304,7 @@ static size_t table_bytes() {
 }
 
 size_t verify_scratch_bytes(const VerifyConfig &cfg, uint32_t n, int num_cus) {
+    // a half-scalar signature needs two tables, in one lane or spread over two
     const size_t per = (cfg.window == 5 ? table_bytes<5>() : table_bytes<4>()) * (cfg.half ? 2 : 1);
     const size_t lanes = cfg.persistent ? (size_t)num_cus * cfg.waves_per_cu * 64u : n;
     return per * (lanes ? lanes : 1);
@@ -175,6 +319,11 @@ static void launch_plain(const VerifyJob &job, uint32_t n, hipStream_t s) {
 template <int W, int HALF>
 static void launch_persistent(const VerifyJob &job, uint32_t waves, hipStream_t s) {
     hipLaunchKernelGGL((verify_persistent_kernel<W, 1, HALF>), dim3(waves), dim3(64), 0, s, job);
+}
+
+template <int W, int WPE>
+static void launch_pair(const VerifyJob &job, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL((verify_pair_kernel<W, WPE>), dim3((n + 31) / 32), dim3(64), 0, s, job, n);
 }
 
 template <int HALF>
@@ -193,6 +342,12 @@ hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint
                              int num_cus, hipStream_t s) {
     if (n_grid == 0) return hipSuccess;
     const bool w5 = cfg.window == 5;
+    if (!cfg.persistent && cfg.lanes == 2) {
+        const bool wpe2 = cfg.waves_per_eu == 2;
+        if (w5) wpe2 ? launch_pair<5, 2>(job, n_grid, s) : launch_pair<5, 1>(job, n_grid, s);
+        else    wpe2 ? launch_pair<4, 2>(job, n_grid, s) : launch_pair<4, 1>(job, n_grid, s);
+        return hipGetLastError();
+    }
     if (!cfg.persistent) {
         if (cfg.half) launch_plain_cfg<1>(cfg, job, n_grid, s);
         else          launch_plain_cfg<0>(cfg, job, n_grid, s);
@@ -217,7 +372,10 @@ hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint
 }
 
 void config_name(const VerifyConfig &c, char *buf, size_t len) {
-    if (c.persistent)
+    if (c.lanes == 2)
+        snprintf(buf, len, "one-shot W=%u scalars=half lanes=2 waves_per_eu=%u", c.window,
+                 c.waves_per_eu);
+    else if (c.persistent)
         snprintf(buf, len, "persistent W=%u scalars=%s sort=%u waves_per_cu=%u", c.window,
                  c.half ? "half" : "full", c.sort, c.waves_per_cu);
     else
@@ -231,14 +389,17 @@ int verify_config_candidates(VerifyConfig *out, int max) {
         for (int w = 4; w <= 5; w++)
             for (int wpe = 1; wpe <= 2; wpe++)
                 if (k < max)
-                    out[k++] = VerifyConfig{0, (uint8_t)w, (uint8_t)wpe, 0, 0, (uint8_t)half};
+                    out[k++] = VerifyConfig{0, (uint8_t)w, (uint8_t)wpe, 0, 0, (uint8_t)half, 1};
         for (int w = 4; w <= 5; w++)
             for (int wpc = 4; wpc <= 8; wpc *= 2)
                 for (int sort = 0; sort <= 1; sort++)
                     if (k < max)
                         out[k++] = VerifyConfig{1, (uint8_t)w, 1, (uint8_t)sort, (uint16_t)wpc,
-                                                (uint8_t)half};
+                                                (uint8_t)half, 1};
     }
+    for (int w = 4; w <= 5; w++)
+        for (int wpe = 1; wpe <= 2; wpe++)
+            if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, (uint8_t)wpe, 0, 0, 1, 2};
     return k;
 }
 
