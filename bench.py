@@ -41,18 +41,19 @@ def verify_ops(lens):
     return float(np.sum(W_VERIFY_128 + (blocks - 2) * W_SHA512))
 
 
-def measured_traffic(launch_config):
-    """HBM bytes per verify launch of the validator batch for `launch_config`,
-    as measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE, calibrated against a copy
-    of known size) and recorded in profiles/verify_poh_sign_bench.md; None when
-    that configuration has no measurement."""
+def measured_row(launch_config):
+    """rocprofv3 measurements of one verify launch of the validator batch in
+    `launch_config`, as recorded in profiles/verify_poh_sign_bench.md: HBM bytes
+    (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors from a copy of known size)
+    and issued VALU lane-ops per signature; None when that configuration has no
+    measurement."""
     path = os.path.join(ROOT, "profiles", "verify_poh_sign_bench.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 5 and cells[0] == launch_config and cells[4].isdigit():
-            return int(cells[4])
+        if len(cells) >= 7 and cells[0] == launch_config and cells[4].isdigit():
+            return {"traffic": int(cells[4]), "valu_lane_ops_per_sig": float(cells[6]) * 1e3}
     return None
 
 
@@ -144,6 +145,61 @@ def served_leg(lib, b, mask, seconds=1.0):
                 "vs_back_to_back": best["verify"]["units_per_s"] / b2b["sig_verifies_per_s"]}}
 
 
+def run_cpu_extra_baselines(seconds=2.0):
+    """The reference CPU path for PoH (scalar SHA-256 chain, poh_verify.cu:20-25
+    over cuda-sha256/sha256.cu) and for sign (sign.cu:9-38), one thread and all
+    host cores, each on a sample of about `seconds` per thread."""
+    sys.path.insert(0, ROOT)
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import RefLib
+    if not RefLib.available():
+        return None
+    ref = RefLib()
+    threads = cpu_threads()
+    out = {"kind": "reference", "cores": threads}
+    # PoH: one chain per thread
+    h = np.zeros(32, np.uint8)
+    t = time.perf_counter()
+    ref.poh_verify_many(h, np.array([20000], np.uint64))
+    per_hash = (time.perf_counter() - t) / 20000
+    reps = int(seconds / per_hash)
+
+    def poh(_):
+        hh = np.zeros(32, np.uint8)
+        ref.poh_verify_many(hh, np.array([reps], np.uint64))
+
+    t = time.perf_counter()
+    poh(0)
+    one = reps / (time.perf_counter() - t)
+    with ThreadPoolExecutor(threads) as ex:
+        t = time.perf_counter()
+        list(ex.map(poh, range(threads)))
+        allc = threads * reps / (time.perf_counter() - t)
+    out["poh_hashes_per_s"] = {"1_thread": one, "all_threads": allc,
+                               "sample": "%d chained SHA-256 of a 32-byte state per thread" % reps}
+    # sign: 128-byte messages
+    pk, sk = ref.create_keypair(bytes(32))
+    msg = bytes(range(128))
+    t = time.perf_counter()
+    ref.sign_repeat(msg, pk, sk, 200)
+    per_sign = (time.perf_counter() - t) / 200
+    k = max(200, int(seconds / per_sign))
+
+    def sign(_):
+        ref.sign_repeat(msg, pk, sk, k)
+
+    t = time.perf_counter()
+    sign(0)
+    one = k / (time.perf_counter() - t)
+    with ThreadPoolExecutor(threads) as ex:
+        t = time.perf_counter()
+        list(ex.map(sign, range(threads)))
+        allc = threads * k / (time.perf_counter() - t)
+    out["sign_per_s"] = {"1_thread": one, "all_threads": allc,
+                         "sample": "%d signatures of a 128-byte message per thread" % k}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,7 +286,9 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
                      "unit": "TOP/s (int32 lane-ops)", "frac": achieved / VALU_PEAK,
-                     "traffic": measured_traffic(eng.config()),
+                     "traffic": (measured_row(eng.config()) or {}).get("traffic"),
+                     "issued_valu_lane_ops_per_sig":
+                         (measured_row(eng.config()) or {}).get("valu_lane_ops_per_sig"),
                      "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch, "
                                        "profiles/verify_poh_sign_bench.md",
                      "kernel": "verify kernel (tuned configuration)",
@@ -345,6 +403,8 @@ def main():
         cb, cpu_mask = run_cpu_baseline(b)
         result["cpu_baseline"] = cb
         result["parity_vs_cpu"] = bool(np.array_equal(cpu_mask, mask[:cpu_mask.size]))
+        if not args.no_extra:
+            result["cpu_baseline_poh_sign"] = run_cpu_extra_baselines()
     print(json.dumps(result), flush=True)
 
 

@@ -160,6 +160,7 @@ class RefLib:
         lib.ref_verify.restype = ctypes.c_int
         lib.ref_create_keypair.argtypes = [_u8p, _u8p, _u8p]
         lib.ref_sign.argtypes = [_u8p, _u8p, ctypes.c_size_t, _u8p, _u8p]
+        lib.ref_sign_repeat.argtypes = [_u8p, _u8p, ctypes.c_size_t, _u8p, _u8p, ctypes.c_uint32]
         lib.ref_sc_reduce.argtypes = [_u8p]
         lib.ref_sc_muladd.argtypes = [_u8p, _u8p, _u8p, _u8p]
         lib.ref_poh.argtypes = [_u8p, _u64p, ctypes.c_size_t]
@@ -198,6 +199,13 @@ class RefLib:
         sig = np.zeros(64, np.uint8)
         m, p, s = _buf(msg), _buf(pk), _buf(sk)
         self.lib.ref_sign(_ptr(sig), _ptr(m), len(m), _ptr(p), _ptr(s))
+        return sig.tobytes()
+
+    def sign_repeat(self, msg, pk, sk, count):
+        """`count` signatures of one message (a timing loop in C)."""
+        sig = np.zeros(64, np.uint8)
+        m, p, s = _buf(msg), _buf(pk), _buf(sk)
+        self.lib.ref_sign_repeat(_ptr(sig), _ptr(m), len(m), _ptr(p), _ptr(s), count)
         return sig.tobytes()
 
     def sc_reduce(self, s64):

@@ -146,6 +146,12 @@ void ref_poh(unsigned char *hashes, const uint64_t *num_hashes, size_t num) {
     }
 }
 
+// `count` signatures of one message (timing loop for the CPU baseline).
+void ref_sign_repeat(unsigned char *sig, const unsigned char *msg, size_t len,
+                     const unsigned char *pk, const unsigned char *sk, uint32_t count) {
+    for (uint32_t i = 0; i < count; i++) ref_sign(sig, msg, len, pk, sk);
+}
+
 void ref_verify_many(const unsigned char *packets, uint32_t n,
                      const uint32_t *message_lens, const uint32_t *pk_off,
                      const uint32_t *sig_off, const uint32_t *msg_off,

@@ -82,7 +82,7 @@ void ed25519_sign_many(const gpu_Elems *elems, uint32_t num_elems, uint32_t mess
     SPL_HIP_TRY(launch_sign(q.d_packets.as<uint8_t>(), pbytes, tab, tab + n, tab + 2ull * n,
                             tab + 3ull * n, n, q.d_out.as<uint8_t>(), l.dev->d_tables, s));
     hout = q.h_stage.as<uint8_t>();
-    SPL_HIP_TRY(hipMemcpyAsync(hout, q.d_out.ptr, 64ull * n, hipMemcpyDeviceToHost, s));
+    SPL_HIP_TRY(launch_copy16(hout, q.d_out.ptr, 64ull * n, s));
     SPL_HIP_TRY(hipStreamSynchronize(s));
     memcpy(signatures_out, hout, 64ull * n);
     release_queue(l);

@@ -53,17 +53,18 @@ int device_cus(int dev) {
 }
 
 // Packs gpu_Elems and the four offset tables into the queue's pinned staging
-// buffer and copies them to the device (the reference's setup_gpu_ctx,
-// gpu_ctx.cu:75-144, with one copy per table instead of one per elem).
+// buffer and copies them to the device with two copy kernels (the reference's
+// setup_gpu_ctx, gpu_ctx.cu:75-144, makes one DMA copy per elem).
 hipError_t stage_packets(QueueCtx &q, const gpu_Elems *elems, uint32_t num_elems,
                          uint32_t message_size, uint64_t packets_bytes, uint32_t n,
                          const uint32_t *t0, const uint32_t *t1, const uint32_t *t2,
                          const uint32_t *t3, hipStream_t s) {
     hipError_t e;
-    uint64_t tab_bytes = 16ull * n;
+    const uint64_t tab_bytes = 16ull * n;
+    const uint64_t tab_at = (packets_bytes + 15) & ~15ull;   // 16-byte aligned for the copy
     if ((e = q.d_packets.reserve(packets_bytes + 16)) != hipSuccess) return e;
     if ((e = q.d_offsets.reserve(tab_bytes)) != hipSuccess) return e;
-    if ((e = q.h_stage.reserve(packets_bytes + tab_bytes + 64 * (uint64_t)n)) != hipSuccess)
+    if ((e = q.h_stage.reserve(tab_at + tab_bytes + 64 * (uint64_t)n + 16)) != hipSuccess)
         return e;
     uint8_t *h = q.h_stage.as<uint8_t>();
     uint64_t cur = 0;
@@ -73,16 +74,14 @@ hipError_t stage_packets(QueueCtx &q, const gpu_Elems *elems, uint32_t num_elems
         if (b) memcpy(h + cur, elems[k].elems, b);
         cur += b;
     }
-    if (cur < packets_bytes) memset(h + cur, 0, packets_bytes - cur);
-    uint32_t *tab = reinterpret_cast<uint32_t *>(h + packets_bytes);
+    if (cur < tab_at) memset(h + cur, 0, tab_at - cur);
+    uint32_t *tab = reinterpret_cast<uint32_t *>(h + tab_at);
     memcpy(tab, t0, 4ull * n);
     memcpy(tab + n, t1, 4ull * n);
     memcpy(tab + 2ull * n, t2, 4ull * n);
     memcpy(tab + 3ull * n, t3, 4ull * n);
-    if ((e = hipMemcpyAsync(q.d_packets.ptr, h, packets_bytes, hipMemcpyHostToDevice, s)) !=
-        hipSuccess)
-        return e;
-    return hipMemcpyAsync(q.d_offsets.ptr, tab, tab_bytes, hipMemcpyHostToDevice, s);
+    if ((e = launch_copy16(q.d_packets.ptr, h, packets_bytes, s)) != hipSuccess) return e;
+    return launch_copy16(q.d_offsets.ptr, tab, tab_bytes, s);
 }
 
 VerifyJob make_verify_job(const uint8_t *packets, uint64_t size, const uint32_t *lens,
@@ -192,7 +191,7 @@ void ed25519_verify_many(const gpu_Elems *elems, uint32_t num_elems, uint32_t me
     VerifyJob job;
     SPL_HIP_TRY(stage_packets(q, elems, num_elems, message_size, pbytes, n, message_lens,
                               public_key_offsets, signature_offsets, message_start_offsets, s));
-    SPL_HIP_TRY(q.d_out.reserve(n));
+    SPL_HIP_TRY(q.d_out.reserve(n + 16));
     SPL_HIP_TRY(q.d_scratch.reserve(verify_scratch_bytes(cfg, n, 0)));
     tab = q.d_offsets.as<uint32_t>();
     job = make_verify_job(q.d_packets.as<uint8_t>(), pbytes, tab, tab + n, tab + 2ull * n,
@@ -200,7 +199,7 @@ void ed25519_verify_many(const gpu_Elems *elems, uint32_t num_elems, uint32_t me
                           l.dev->d_tables);
     SPL_HIP_TRY(launch_verify_cfg(cfg, job, n, 0, s));
     hout = q.h_stage.as<uint8_t>();
-    SPL_HIP_TRY(hipMemcpyAsync(hout, q.d_out.ptr, n, hipMemcpyDeviceToHost, s));
+    SPL_HIP_TRY(launch_copy16(hout, q.d_out.ptr, n, s));
     SPL_HIP_TRY(hipStreamSynchronize(s));
     memcpy(out, hout, n);
     release_queue(l);

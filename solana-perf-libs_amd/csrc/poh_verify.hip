@@ -58,15 +58,15 @@ int poh_verify_many(uint8_t *hashes, const uint64_t *num_hashes_arr, size_t num_
     hipStream_t s = use_non_default_stream ? q.stream : nullptr;
     size_t hb = num_elems * 32, cb = num_elems * 8;
     SPL_HIP_TRY(q.d_hashes.reserve(hb));
-    SPL_HIP_TRY(q.d_counts.reserve(cb));
-    SPL_HIP_TRY(q.h_stage.reserve(hb + cb));
+    SPL_HIP_TRY(q.d_counts.reserve(cb + 16));
+    SPL_HIP_TRY(q.h_stage.reserve(hb + cb + 16));
     memcpy(q.h_stage.as<uint8_t>(), hashes, hb);
     memcpy(q.h_stage.as<uint8_t>() + hb, num_hashes_arr, cb);
-    SPL_HIP_TRY(hipMemcpyAsync(q.d_hashes.ptr, q.h_stage.ptr, hb, hipMemcpyHostToDevice, s));
-    SPL_HIP_TRY(hipMemcpyAsync(q.d_counts.ptr, q.h_stage.as<uint8_t>() + hb, cb,
-                               hipMemcpyHostToDevice, s));
+    // copy kernels, not DMA copies (see launch_copy16)
+    SPL_HIP_TRY(launch_copy16(q.d_hashes.ptr, q.h_stage.ptr, hb, s));
+    SPL_HIP_TRY(launch_copy16(q.d_counts.ptr, q.h_stage.as<uint8_t>() + hb, cb, s));
     SPL_HIP_TRY(launch_poh(q.d_hashes.as<uint32_t>(), q.d_counts.as<uint64_t>(), num_elems, s));
-    SPL_HIP_TRY(hipMemcpyAsync(q.h_stage.ptr, q.d_hashes.ptr, hb, hipMemcpyDeviceToHost, s));
+    SPL_HIP_TRY(launch_copy16(q.h_stage.ptr, q.d_hashes.ptr, hb, s));
     SPL_HIP_TRY(hipStreamSynchronize(s));
     memcpy(hashes, q.h_stage.ptr, hb);
     rc = 0;

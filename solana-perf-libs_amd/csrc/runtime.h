@@ -101,6 +101,11 @@ const ed25519_tables *device_tables(int hip_device);
 // Returns nullptr on allocation failure.
 void *stream_scratch(int hip_device, hipStream_t stream, size_t bytes);
 
+// Copies `bytes` rounded up to 16 from src to dst on stream s with a kernel
+// (device or pinned host memory; both 16-byte aligned, both valid for the
+// rounded length).  Used instead of DMA copies for host<->device transfers.
+hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t s);
+
 // Enqueues SHA-256 chains: hashes[8*i..8*i+8) <- SHA-256^counts[i] of itself,
 // in place, for i < n (poh_verify.hip).
 hipError_t launch_poh(uint32_t *d_hashes, const uint64_t *d_counts, uint64_t n,

@@ -139,31 +139,6 @@ uint64_t verify_block_bytes(uint64_t packets_bytes, uint64_t n) {
 }
 uint64_t poh_block_bytes(uint64_t n) { return align_up(40 * n, 256); }
 
-// Copies n16 16-byte words.  The server moves batches between its pinned
-// mirror and the device ring with this kernel rather than with DMA copies:
-// on the MI355X, hipMemcpyAsync between pinned host memory and the device
-// was measured to stall for about 7 ms now and then (the same 1 MiB copy
-// otherwise takes 0.1 ms), while this kernel, reading or writing the pinned
-// memory over PCIe, took 0.27 ms every time.
-__global__ void __launch_bounds__(256) copy16_kernel(uint4 *__restrict__ dst,
-                                                     const uint4 *__restrict__ src,
-                                                     uint64_t n16) {
-    raise_wave_priority();
-    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += step)
-        dst[i] = src[i];
-}
-
-// Both pointers 16-byte aligned; copies bytes rounded up to 16.
-hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t s) {
-    const uint64_t n16 = (bytes + 15) / 16;
-    if (n16 == 0) return hipSuccess;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 1024);
-    hipLaunchKernelGGL(copy16_kernel, dim3(grid), dim3(256), 0, s, static_cast<uint4 *>(dst),
-                       static_cast<const uint4 *>(src), n16);
-    return hipGetLastError();
-}
-
 struct LatencyStats {
     std::vector<uint64_t> ns;             // per request: submit -> results delivered
     // per batch: formed -> enqueued (host staging), enqueued -> completion
