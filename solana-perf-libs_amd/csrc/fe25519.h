@@ -80,15 +80,31 @@ SPL_HD void carry_round(T c[10], int i) {
     else c[i + 1] += cy;
 }
 
-// 64-bit column sums -> tight limbs.  Two interleaved carry chains
-// (0..4 and 4..9) halve the dependency depth.
+// Rounding offset of column i, 2^(b-1).  Column sums are started at this
+// value instead of 0 (a free initial accumulator for the multiply-add
+// chains), so the first carry out of each column needs no add of its own.
+#define SPL_COL_BIAS(i) ((int64_t)1 << (SPL_LIMB_BITS(i) - 1))
+
+// First carry of a biased column (c[i] = value + 2^(b-1)): the rounded carry
+// is c[i] >> b and the limb is (c[i] mod 2^b) - 2^(b-1), in [-2^(b-1), 2^(b-1)).
+SPL_HD void carry_biased(int64_t c[10], int i) {
+    const int b = SPL_LIMB_BITS(i);
+    const int64_t cy = c[i] >> b;
+    c[i] = (c[i] & (((int64_t)1 << b) - 1)) - ((int64_t)1 << (b - 1));
+    if (i == 9) c[0] += 19 * cy;
+    else c[i + 1] += cy;
+}
+
+// Biased 64-bit column sums (SPL_COL_BIAS added) -> tight limbs.  Two
+// interleaved carry chains (0..4 and 4..9) halve the dependency depth; limbs
+// 4 and 0 are carried a second time, unbiased by then.
 SPL_HD void fe_reduce_cols(fe &h, int64_t c[10]) {
-    carry_round(c, 0); carry_round(c, 4);
-    carry_round(c, 1); carry_round(c, 5);
-    carry_round(c, 2); carry_round(c, 6);
-    carry_round(c, 3); carry_round(c, 7);
-    carry_round(c, 4); carry_round(c, 8);
-    carry_round(c, 9);
+    carry_biased(c, 0); carry_biased(c, 4);
+    carry_biased(c, 1); carry_biased(c, 5);
+    carry_biased(c, 2); carry_biased(c, 6);
+    carry_biased(c, 3); carry_biased(c, 7);
+    carry_round(c, 4); carry_biased(c, 8);
+    carry_biased(c, 9);
     carry_round(c, 0);
 #pragma unroll
     for (int i = 0; i < 10; i++) h.v[i] = (int32_t)c[i];
@@ -119,7 +135,7 @@ SPL_HD void fe_mul(fe &h, const fe &f, const fe &g) {
     int64_t c[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) {
-        int64_t s = 0;
+        int64_t s = SPL_COL_BIAS(k);
 #pragma unroll
         for (int i = 0; i < 10; i++) {
             int j = k - i;
@@ -134,10 +150,11 @@ SPL_HD void fe_mul(fe &h, const fe &f, const fe &g) {
     fe_reduce_cols(h, c);
 }
 
-// Column sums of f^2: each unordered pair once, the factor 2 of the cross
-// terms on the left operand and the odd-odd factor 2 / wrap factor 19 on the
-// right one (38 f_j only for odd, 25-bit, limbs so it fits in 31 bits).
-SPL_HD void fe_sq_cols(int64_t c[10], const fe &f) {
+// Column sums of f^2 plus SPL_COL_BIAS >> bias_shift: each unordered pair
+// once, the factor 2 of the cross terms on the left operand and the odd-odd
+// factor 2 / wrap factor 19 on the right one (38 f_j only for odd, 25-bit,
+// limbs so it fits in 31 bits).
+SPL_HD void fe_sq_cols(int64_t c[10], const fe &f, int bias_shift = 0) {
     int32_t f2[10], f19[10], f38[10];
 #pragma unroll
     for (int i = 0; i < 10; i++) {
@@ -146,7 +163,7 @@ SPL_HD void fe_sq_cols(int64_t c[10], const fe &f) {
         f38[i] = 38 * f.v[i];
     }
 #pragma unroll
-    for (int k = 0; k < 10; k++) c[k] = 0;
+    for (int k = 0; k < 10; k++) c[k] = SPL_COL_BIAS(k) >> bias_shift;
 #pragma unroll
     for (int i = 0; i < 10; i++) {
 #pragma unroll
@@ -168,10 +185,10 @@ SPL_HD void fe_sq(fe &h, const fe &f) {
     fe_reduce_cols(h, c);
 }
 
-// h = 2 f^2
+// h = 2 f^2 (half the bias before doubling: the doubled sums carry it whole)
 SPL_HD void fe_sq2(fe &h, const fe &f) {
     int64_t c[10];
-    fe_sq_cols(c, f);
+    fe_sq_cols(c, f, 1);
 #pragma unroll
     for (int k = 0; k < 10; k++) c[k] += c[k];
     fe_reduce_cols(h, c);
