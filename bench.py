@@ -114,10 +114,17 @@ def served_leg(lib, b, mask, seconds=1.0):
     rng = np.random.default_rng(12)
     steps = []
     server = perf_libs.Server(lib, hbm_budget_bytes=1 << 30)
+    error = None
     try:
         for rate in (1e5, 2e5, 5e5, 1e6, 2e6, 4e6, 8e6, 16e6):
             trace = loadgen.make_trace(pools, rng, rate, seconds)
-            recs, span, _ = loadgen.run_served(server, trace)
+            try:
+                recs, span, _ = loadgen.run_served(server, trace)
+            except loadgen.ServedTimeout as exc:
+                error = str(exc)
+                log("bench: served leg stopped: " + error)
+                server.abandon()
+                break
             row = {"offered_sigs_per_s": rate, **loadgen.summarize(recs, span),
                    "server": server.stats(reset=True)}
             steps.append(row)
@@ -132,6 +139,8 @@ def served_leg(lib, b, mask, seconds=1.0):
     b2b = loadgen.run_back_to_back(lib, trace)
     log(f"bench: back-to-back plain calls: {b2b['sig_verifies_per_s']:.3e} sig/s, "
         f"p99 {b2b['p99_ms']:.2f} ms")
+    if error is not None:
+        return {"error": error, "steps": steps}
     ok = [r for r in steps if r["verify"]["p99_ms"] <= 2.0]
     best = max(ok, key=lambda r: r["verify"]["units_per_s"]) if ok else None
     return {"workload": "Poisson verify requests (log-uniform 1-2048 packets of the validator "

@@ -378,40 +378,45 @@ struct spl_server {
         done_cv.notify_all();
     }
 
+    // The loop thread owns the slots; mu guards the pending queues, the
+    // request records and the ring, and is taken only to move work between
+    // them, never while polling events, so submitters and waiters are not
+    // starved by the poll.
     void run() {
         (void)hipSetDevice(dev);
-        std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             bool progress = false;
             for (Slot &s : slots) {
                 if (!s.batch) continue;
-                hipError_t q = hipEventQuery(s.done);
+                const hipError_t q = hipEventQuery(s.done);
                 if (q == hipErrorNotReady) continue;
                 Batch *b = s.batch;
                 b->t_device_done = now_ns();
                 s.batch = nullptr;
-                inflight--;
                 if (q != hipSuccess)
                     fprintf(stderr, "spl_server: batch of %u failed: %s\n", b->n,
                             hipGetErrorString(q));
-                lk.unlock();
                 deliver(*b, q == hipSuccess);
-                lk.lock();
+                std::lock_guard<std::mutex> g(mu);
+                inflight--;
                 complete(b, q == hipSuccess);
                 progress = true;
             }
             for (Slot &s : slots) {
-                if (s.batch || pending[s.kind].empty()) continue;
-                Batch *b = form_batch(s.kind);
+                if (s.batch) continue;
+                Batch *b = nullptr;
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (!pending[s.kind].empty()) b = form_batch(s.kind);
+                }
                 if (!b) continue;
-                lk.unlock();
                 const hipError_t e = launch(s, *b);
                 if (e != hipSuccess) {
                     fprintf(stderr, "spl_server: launch of a batch of %u failed: %s\n", b->n,
                             hipGetErrorString(e));
                     deliver(*b, false);
                 }
-                lk.lock();
+                std::lock_guard<std::mutex> g(mu);
                 if (e == hipSuccess) {
                     s.batch = b;
                     inflight++;
@@ -425,23 +430,25 @@ struct spl_server {
             // Busy-poll rather than sleep: a sleeping thread's core drops into
             // an idle state, and waking it (and the memcpy that follows) was
             // measured to add up to a millisecond at moderate request rates.
-            if (inflight == 0) {
-                if (stop && pending[0].empty() && pending[1].empty()) break;
-                // idle: keep polling for a while before blocking
-                const uint64_t seen = submitted.load(std::memory_order_relaxed);
-                const uint64_t until = now_ns() + kIdleSpinNs;
-                lk.unlock();
-                while (submitted.load(std::memory_order_relaxed) == seen && now_ns() < until)
-                    std::this_thread::yield();
-                lk.lock();
-                wake.wait(lk, [this] {
-                    return stop || !pending[0].empty() || !pending[1].empty();
-                });
-            } else {
-                lk.unlock();
-                std::this_thread::yield();
-                lk.lock();
+            bool idle;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                idle = inflight == 0;
+                if (idle && stop && pending[0].empty() && pending[1].empty()) break;
             }
+            if (!idle) {
+                std::this_thread::yield();
+                continue;
+            }
+            // idle: keep polling for a while before blocking
+            const uint64_t seen = submitted.load(std::memory_order_relaxed);
+            const uint64_t until = now_ns() + kIdleSpinNs;
+            while (submitted.load(std::memory_order_relaxed) == seen && now_ns() < until)
+                std::this_thread::yield();
+            std::unique_lock<std::mutex> lk(mu);
+            wake.wait(lk, [this] {
+                return stop || !pending[0].empty() || !pending[1].empty();
+            });
         }
     }
 

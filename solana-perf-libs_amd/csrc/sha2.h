@@ -21,6 +21,39 @@ SPL_HD uint32_t rotr32(uint32_t x, int n) {
 
 SPL_HD uint64_t rotr64(uint64_t x, int n) { return (x >> n) | (x << (64 - n)); }
 
+// a ^ b ^ c in one V_BITOP3_B32 (truth table 0x96) on gfx950; the compiler
+// otherwise emits two V_XOR_B32 for each of the Sigma/sigma functions.
+// Measured on the PoH loop: 1552 -> 1402 instructions per SHA-256, and
+// 1344 with the majority below (the compiler then also forms Ch with it).
+SPL_HD uint32_t xor3_32(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
+// Majority of three (truth table 0xe8, symmetric in its inputs).
+SPL_HD uint32_t maj32(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xe8);
+#else
+    return (a & b) | (c & (a | b));
+#endif
+}
+
+SPL_HD uint64_t maj64(uint64_t a, uint64_t b, uint64_t c) {
+    return ((uint64_t)maj32((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32))
+            << 32) |
+           maj32((uint32_t)a, (uint32_t)b, (uint32_t)c);
+}
+
+SPL_HD uint64_t xor3_64(uint64_t a, uint64_t b, uint64_t c) {
+    return ((uint64_t)xor3_32((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32))
+            << 32) |
+           xor3_32((uint32_t)a, (uint32_t)b, (uint32_t)c);
+}
+
 SPL_HD uint32_t bswap32(uint32_t x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_perm(0, x, 0x00010203u);
@@ -71,16 +104,16 @@ SPL_HD void sha256_32byte(uint32_t st[8]) {
             wt = w[t];
         } else {
             uint32_t w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
-            uint32_t s0 = rotr32(w15, 7) ^ rotr32(w15, 18) ^ (w15 >> 3);
-            uint32_t s1 = rotr32(w2, 17) ^ rotr32(w2, 19) ^ (w2 >> 10);
+            uint32_t s0 = xor3_32(rotr32(w15, 7), rotr32(w15, 18), w15 >> 3);
+            uint32_t s1 = xor3_32(rotr32(w2, 17), rotr32(w2, 19), w2 >> 10);
             wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
             w[t & 15] = wt;
         }
-        uint32_t S1 = rotr32(e, 6) ^ rotr32(e, 11) ^ rotr32(e, 25);
+        uint32_t S1 = xor3_32(rotr32(e, 6), rotr32(e, 11), rotr32(e, 25));
         uint32_t ch = g ^ (e & (f ^ g));
         uint32_t t1 = h + S1 + ch + K[t] + wt;
-        uint32_t S0 = rotr32(a, 2) ^ rotr32(a, 13) ^ rotr32(a, 22);
-        uint32_t maj = (a & b) | (c & (a | b));
+        uint32_t S0 = xor3_32(rotr32(a, 2), rotr32(a, 13), rotr32(a, 22));
+        uint32_t maj = maj32(a, b, c);
         uint32_t t2 = S0 + maj;
         h = g; g = f; f = e; e = d + t1;
         d = c; c = b; b = a; a = t1 + t2;
@@ -140,16 +173,16 @@ SPL_HD void sha512_block(uint64_t st[8], uint64_t w[16]) {
             wt = w[t];
         } else {
             uint64_t w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
-            uint64_t s0 = rotr64(w15, 1) ^ rotr64(w15, 8) ^ (w15 >> 7);
-            uint64_t s1 = rotr64(w2, 19) ^ rotr64(w2, 61) ^ (w2 >> 6);
+            uint64_t s0 = xor3_64(rotr64(w15, 1), rotr64(w15, 8), w15 >> 7);
+            uint64_t s1 = xor3_64(rotr64(w2, 19), rotr64(w2, 61), w2 >> 6);
             wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
             w[t & 15] = wt;
         }
-        uint64_t S1 = rotr64(e, 14) ^ rotr64(e, 18) ^ rotr64(e, 41);
+        uint64_t S1 = xor3_64(rotr64(e, 14), rotr64(e, 18), rotr64(e, 41));
         uint64_t ch = g ^ (e & (f ^ g));
         uint64_t t1 = h + S1 + ch + K[t] + wt;
-        uint64_t S0 = rotr64(a, 28) ^ rotr64(a, 34) ^ rotr64(a, 39);
-        uint64_t maj = (a & b) | (c & (a | b));
+        uint64_t S0 = xor3_64(rotr64(a, 28), rotr64(a, 34), rotr64(a, 39));
+        uint64_t maj = maj64(a, b, c);
         uint64_t t2 = S0 + maj;
         h = g; g = f; f = e; e = d + t1;
         d = c; c = b; b = a; a = t1 + t2;

@@ -105,7 +105,11 @@ def make_trace(pools, rng, sig_rate, duration_s, poh_rate=50.0, max_packets=2048
     return trace
 
 
-def run_served(server, trace, lead_s=0.05):
+class ServedTimeout(RuntimeError):
+    """A request did not complete within the wait limit."""
+
+
+def run_served(server, trace, lead_s=0.05, wait_ms=60000):
     """Submits every request at its arrival time (open loop) and waits for all
     of them.  Returns per-request records: kind, units, latency from the
     scheduled arrival to completion (ms), submit lateness (ms), and whether the
@@ -132,7 +136,10 @@ def run_served(server, trace, lead_s=0.05):
     recs = []
     for t, target, kind, req, res in tickets:
         assert t > 0, "submit refused"
-        rc, (ts, ta, td) = server.wait(t)
+        rc, (ts, ta, td) = server.wait(t, wait_ms)
+        if rc == 1:
+            raise ServedTimeout(f"request {t} ({kind}) not complete after {wait_ms} ms; "
+                                f"server stats {server.stats()}")
         if kind == "verify":
             exp = req[5]
             ok = rc == 0 and np.array_equal(res[:exp.size], exp)

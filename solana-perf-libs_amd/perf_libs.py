@@ -330,6 +330,14 @@ class Server:
             self.h = None
             self._live.clear()
 
+    def abandon(self):
+        """Drop the handle without spl_server_destroy (which waits for every
+        submitted request): for a server whose work did not complete.  The
+        request buffers are kept alive with it."""
+        if self.h:
+            _ABANDONED.append((self.h, self._live))
+            self.h = None
+
     def __del__(self):
         try:
             self.close()
@@ -338,6 +346,7 @@ class Server:
 
 
 _LIB = None
+_ABANDONED = []
 
 
 def load():
