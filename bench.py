@@ -209,6 +209,34 @@ def run_cpu_extra_baselines(seconds=2.0):
     return out
 
 
+def run_cpu_multibuffer_poh(seconds=2.0):
+    """This build's CPU PoH (libpoh-simd.so, the multi-buffer form of the
+    reference's poh-simd ISPC library, poh-verify.ispc:3-179): 16 chains per
+    call, one thread and all host cores, about `seconds` per thread."""
+    import poh_simd
+    try:
+        simd = poh_simd.load()
+    except (OSError, FileNotFoundError):
+        return None
+    threads = cpu_threads()
+    h = np.zeros(32 * 16, np.uint8)
+    t = time.perf_counter()
+    simd.many(h, np.full(16, 2000, np.uint64), 1)
+    per_hash = (time.perf_counter() - t) / (16 * 2000)
+    reps = max(2000, int(seconds / per_hash / 16))
+    one_h = np.zeros(32 * 16, np.uint8)
+    t = time.perf_counter()
+    simd.many(one_h, np.full(16, reps, np.uint64), 1)
+    one = 16 * reps / (time.perf_counter() - t)
+    all_h = np.zeros(32 * 16 * threads, np.uint8)
+    t = time.perf_counter()
+    simd.many(all_h, np.full(16 * threads, reps, np.uint64), threads)
+    allc = 16 * threads * reps / (time.perf_counter() - t)
+    return {"kind": "port", "cores": threads, "kernel": simd.kernel(),
+            "1_thread": one, "all_threads": allc,
+            "sample": "16 chains per thread x %d chained SHA-256 of a 32-byte state" % reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,6 +442,7 @@ def main():
         result["parity_vs_cpu"] = bool(np.array_equal(cpu_mask, mask[:cpu_mask.size]))
         if not args.no_extra:
             result["cpu_baseline_poh_sign"] = run_cpu_extra_baselines()
+            result["cpu_poh_multibuffer"] = run_cpu_multibuffer_poh()
     print(json.dumps(result), flush=True)
 
 
