@@ -204,6 +204,54 @@ int64_t spl_server_submit_poh(spl_server *server, uint8_t *hashes, const uint64_
 int spl_server_wait(spl_server *server, int64_t ticket, int timeout_ms, uint64_t *times_ns);
 int spl_server_stats(spl_server *server, char *buf, size_t len, int reset);
 
+/* ------------------------------------------------------------------------
+ * Extension: packets in, verdicts out (no counterpart in the reference, which
+ * consumes the four offset tables, gpu_ctx.cu:133-136, and leaves building
+ * them to its caller).  The caller's host pass over Solana wire-format packets
+ * (agave perf/src/sigverify.rs: generate_offsets / do_get_packet_offsets,
+ * and copy_return_values afterwards) runs on the device instead.
+ *
+ * Packets are num_packets slots of packet_stride bytes (< 4 GiB in all);
+ * packet i holds packet_sizes[i] valid bytes.  A packet gets its signatures
+ * when: size >= 65 and size <= packet_stride; a short-vec signature count n
+ * (u16 in 1-3 bytes, shortest form) is followed by n x 64 signature bytes and
+ * at least one more byte; an optional 0x80 prefix (message version 0; other
+ * versions fail) precedes a 3-byte header whose first byte equals n and is
+ * above its second; a short-vec account-key count k >= n follows, and the k x
+ * 32 key bytes fit in the packet.  Signature j of such a packet is checked
+ * against key j over the message from its first byte (the prefix, if any) to
+ * the end of the packet.  Otherwise the packet has no signatures.
+ *
+ * spl_packet_offsets_device writes, per packet, its signature count and the
+ * row of its first signature (d_sig_lens, d_sig_base), and one row per
+ * signature of the four tables ed25519_verify_many_device takes (at most
+ * max_signatures rows; further rows are not written); d_counts[0] receives
+ * the batch's signature count and d_counts[1] the rows written.
+ * spl_packet_verdicts_device writes d_packet_ok[i] = 1 when packet i has at
+ * least one signature and all of them verified in d_sig_out (num_signatures
+ * entries), else 0.  spl_verify_packets_device does both with the verify in
+ * between, on one stream with no host synchronisation (d_sig_lens may be
+ * NULL), and spl_verify_packets is its synchronous host form, taking the
+ * packets as gpu_Elems like ed25519_verify_many.  All return 0 ok, 1 no GPU
+ * (spl_verify_packets then writes 0 verdicts), 2 bad arguments, 3 device
+ * error. */
+int spl_packet_offsets_device(const uint8_t *d_packets, uint32_t num_packets,
+                              uint32_t packet_stride, const uint32_t *d_packet_sizes,
+                              uint32_t max_signatures, uint32_t *d_sig_lens,
+                              uint32_t *d_sig_base, uint32_t *d_message_lens,
+                              uint32_t *d_public_key_offsets, uint32_t *d_signature_offsets,
+                              uint32_t *d_message_start_offsets, uint32_t *d_counts,
+                              void *stream);
+int spl_packet_verdicts_device(const uint32_t *d_sig_lens, const uint32_t *d_sig_base,
+                               uint32_t num_packets, const uint8_t *d_sig_out,
+                               uint32_t num_signatures, uint8_t *d_packet_ok, void *stream);
+int spl_verify_packets_device(const uint8_t *d_packets, uint32_t num_packets,
+                              uint32_t packet_stride, const uint32_t *d_packet_sizes,
+                              uint8_t *d_packet_ok, uint32_t *d_sig_lens, void *stream);
+int spl_verify_packets(const gpu_Elems *elems, uint32_t num_elems, uint32_t packet_stride,
+                       uint32_t total_packets, const uint32_t *packet_sizes, uint8_t *packet_ok,
+                       uint32_t *sig_lens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,8 @@ EXPORTS = (
     "spl_verify_engine_config", "spl_verify_engine_candidates", "spl_verify_engine_use",
     "spl_server_create", "spl_server_destroy",
     "spl_server_submit_verify", "spl_server_submit_poh", "spl_server_wait", "spl_server_stats",
+    "spl_packet_offsets_device", "spl_packet_verdicts_device", "spl_verify_packets_device",
+    "spl_verify_packets",
 )
 
 
@@ -117,6 +119,16 @@ class PerfLibs:
         lib.spl_server_wait.restype = ctypes.c_int
         lib.spl_server_stats.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int]
         lib.spl_server_stats.restype = ctypes.c_int
+        u32 = ctypes.c_uint32
+        lib.spl_packet_offsets_device.argtypes = [_vp, u32, u32, _vp, u32] + [_vp] * 7 + [_vp]
+        lib.spl_packet_offsets_device.restype = ctypes.c_int
+        lib.spl_packet_verdicts_device.argtypes = [_vp, _vp, u32, _vp, u32, _vp, _vp]
+        lib.spl_packet_verdicts_device.restype = ctypes.c_int
+        lib.spl_verify_packets_device.argtypes = [_vp, u32, u32, _vp, _vp, _vp, _vp]
+        lib.spl_verify_packets_device.restype = ctypes.c_int
+        lib.spl_verify_packets.argtypes = [ctypes.POINTER(GpuElems), u32, u32, u32, _u32p, _u8p,
+                                           _u32p]
+        lib.spl_verify_packets.restype = ctypes.c_int
 
     # ------------------------------------------------------------ host API
     def init(self):
@@ -190,6 +202,26 @@ class PerfLibs:
            *[_p(t, _u32p) for t in tabs], _p(out), nds)
         return out[: n * out_width]
 
+    def verify_packets(self, packet_batches, packet_stride, packet_sizes):
+        """spl_verify_packets: wire-format packets (gpu_Elems list of whole
+        slots) and their lengths in; (rc, verdict per packet, signatures per
+        packet) out."""
+        batches = [np.ascontiguousarray(b, dtype=np.uint8) for b in packet_batches]
+        elems = (GpuElems * max(1, len(batches)))()
+        total = 0
+        for k, b in enumerate(batches):
+            assert b.size % packet_stride == 0
+            elems[k].elems = _p(b)
+            elems[k].num = b.size // packet_stride
+            total += b.size // packet_stride
+        sizes = np.ascontiguousarray(packet_sizes, dtype=np.uint32)
+        assert sizes.size == total
+        ok = np.zeros(max(1, total), np.uint8)
+        lens = np.zeros(max(1, total), np.uint32)
+        rc = self.lib.spl_verify_packets(elems, len(batches), packet_stride, total,
+                                         _p(sizes, _u32p), _p(ok), _p(lens, _u32p))
+        return rc, ok[:total], lens[:total]
+
     def poh_verify_many(self, hashes, num_hashes, use_non_default_stream=1):
         """hashes: uint8 [n*32], updated in place. Returns the ABI's int."""
         nh = np.ascontiguousarray(num_hashes, dtype=np.uint64)
@@ -209,6 +241,22 @@ class PerfLibs:
 
     def poh_verify_many_device(self, d_hashes, d_counts, n, stream=0):
         return self.lib.poh_verify_many_device(d_hashes, d_counts, n, stream)
+
+    def packet_offsets_device(self, d_packets, num, stride, d_sizes, max_sigs, d_sig_lens,
+                              d_sig_base, d_lens, d_pk, d_sig, d_msg, d_counts, stream=0):
+        return self.lib.spl_packet_offsets_device(d_packets, num, stride, d_sizes, max_sigs,
+                                                  d_sig_lens, d_sig_base, d_lens, d_pk, d_sig,
+                                                  d_msg, d_counts, stream)
+
+    def packet_verdicts_device(self, d_sig_lens, d_sig_base, num, d_sig_out, num_sigs, d_ok,
+                               stream=0):
+        return self.lib.spl_packet_verdicts_device(d_sig_lens, d_sig_base, num, d_sig_out,
+                                                   num_sigs, d_ok, stream)
+
+    def verify_packets_device(self, d_packets, num, stride, d_sizes, d_ok, d_sig_lens=None,
+                              stream=0):
+        return self.lib.spl_verify_packets_device(d_packets, num, stride, d_sizes, d_ok,
+                                                  d_sig_lens, stream)
 
 
 class VerifyEngine:

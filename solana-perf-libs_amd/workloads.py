@@ -17,6 +17,12 @@ Workload shapes (signatures sign the message bytes they are checked against):
                           lengths U[max(100, 3+32n), min(1200, 1231-64n)];
                           corruption mix: bit-flipped signatures, S := S + L
                           (non-canonical), public key := an order-<=8 point.
+  wire_packets(...)       Solana wire-format transactions, one per packet
+                          (legacy or version-0 messages), for the packet
+                          parser: short-vec signature count, signatures,
+                          message header, short-vec key count, keys, recent
+                          blockhash, one instruction; valid packets plus
+                          malformed ones that break one parsing rule each.
   poh_chains(n, hashes)   n random 32-byte states, one count per chain.
 """
 import numpy as np
@@ -196,3 +202,158 @@ def poh_chains(n, num_hashes, seed=2):
     counts = np.full(n, num_hashes, np.uint64) if np.isscalar(num_hashes) else \
         np.asarray(num_hashes, np.uint64)
     return hashes, counts
+
+
+def short_u16(v):
+    """Short-vec encoding of a u16: 7 bits per byte, low group first."""
+    out = bytearray()
+    while True:
+        b, v = v & 0x7F, v >> 7
+        out.append(b | (0x80 if v else 0))
+        if not v:
+            return bytes(out)
+
+
+# Ways a packet can break exactly one parsing rule (wire_packets' malformed mix).
+MALFORMED_KINDS = ("short", "oversize", "count_alias", "count_too_big", "version",
+                   "payer_readonly", "count_mismatch", "keys_fewer", "keys_overrun",
+                   "keys_alias", "header_cut")
+
+
+def wire_packets(sign_fn, keypair_fn, seed=7, num_batches=128, max_packets=256, max_sigs=4,
+                 stride=1232, v0_frac=0.5, flip_frac=0.05, noncanon_frac=0.01,
+                 small_order_frac=0.01, malformed_frac=0.02, minimal_frac=0.02,
+                 num_keys=4096):
+    """Solana wire-format packets in batches of U[1, max_packets].
+
+    Packet = short_u16(n) || n signatures || message, message = [0x80 for
+    version 0] || header (n, readonly signed < n, readonly unsigned) ||
+    short_u16(k) || k keys (the n signers first) || 32-byte blockhash || one
+    instruction (|| an empty lookup-table list for version 0).  Messages are
+    sized U[100, min(1200, stride - 1 - 64 n)] like the validator-shaped
+    batch; `minimal_frac` of packets carry only header and keys (the keys end
+    exactly at the packet end).  Signatures cover the message bytes and get
+    the same corruption mix as verify_validator.  `malformed_frac` of packets
+    then break one parsing rule (MALFORMED_KINDS) and must get no signatures.
+
+    Returns a Batch whose tables hold the rows the parser must produce (known
+    from where the builder put each field), with meta: sizes (uint32 per
+    packet), sig_lens (expected signatures per packet), kinds (per row),
+    malformed (per packet: kind or None), batch_counts."""
+    rng = np.random.default_rng(seed)
+    key_seeds = _seeds(rng, num_keys)
+    keys = [keypair_fn(key_seeds[k].tobytes()) for k in range(num_keys)]
+    counts = [int(c) for c in rng.integers(1, max_packets + 1, size=num_batches)]
+    total = sum(counts)
+    pkts = np.zeros((total, stride), np.uint8)
+    sizes = np.zeros(total, np.uint32)
+    sig_lens = np.zeros(total, np.uint32)
+    malformed = [None] * total
+    lens, pko, sgo, mso, kinds = [], [], [], [], []
+    for p in range(total):
+        ns = int(rng.integers(1, max_sigs + 1))
+        v0 = rng.random() < v0_frac
+        minimal = rng.random() < minimal_frac
+        extra = 0 if minimal else int(rng.integers(1, 4))
+        k = ns + extra
+        who = rng.integers(0, num_keys, size=ns)
+        sig_kind, signer_pks = [], []
+        for j in range(ns):
+            pk = keys[who[j]][0]
+            u = rng.random()
+            if u < small_order_frac:
+                pk = SMALL_ORDER_KEYS[int(rng.integers(0, len(SMALL_ORDER_KEYS)))]
+                sig_kind.append("small_order_key")
+            elif u < small_order_frac + noncanon_frac:
+                sig_kind.append("noncanonical_s")
+            elif u < small_order_frac + noncanon_frac + flip_frac:
+                sig_kind.append("bit_flip")
+            else:
+                sig_kind.append("valid")
+            signer_pks.append(pk)
+        prefix = b"\x80" if v0 else b""
+        header = bytes([ns, int(rng.integers(0, ns)), int(rng.integers(0, extra + 1))])
+        key_bytes = b"".join(signer_pks) + rng.integers(0, 256, 32 * extra, np.uint8).tobytes()
+        head = prefix + header + short_u16(k) + key_bytes
+        if minimal:
+            msg = head
+        else:
+            lo = max(100, len(head) + 32 + 8)
+            hi = min(1200, stride - 1 - 64 * ns)
+            target = int(rng.integers(lo, max(lo, hi) + 1))
+            fixed = len(head) + 32 + 1 + 1 + 1 + 2 + (1 if v0 else 0)
+            dlen = max(0, target - fixed - 1)
+            if len(short_u16(dlen)) == 2:
+                dlen = max(0, dlen - 1)
+            ix = (bytes([k - 1]) + short_u16(2) + bytes([0, 1 % k]) + short_u16(dlen)
+                  + rng.integers(0, 256, dlen, np.uint8).tobytes())
+            msg = (head + rng.integers(0, 256, 32, np.uint8).tobytes() + short_u16(1) + ix
+                   + (short_u16(0) if v0 else b""))
+        a = len(short_u16(ns))
+        m = a + 64 * ns
+        size = m + len(msg)
+        assert size <= stride, (size, stride)
+        row = pkts[p]
+        row[:a] = np.frombuffer(short_u16(ns), np.uint8)
+        row[m:size] = np.frombuffer(msg, np.uint8)
+        for j in range(ns):
+            _, sk = keys[who[j]]
+            sig = bytearray(sign_fn(msg, keys[who[j]][0], sk))
+            if sig_kind[j] == "noncanonical_s":
+                sig[32:] = (int.from_bytes(sig[32:], "little") + L_ORDER).to_bytes(32, "little")
+            elif sig_kind[j] == "bit_flip":
+                bit = int(rng.integers(0, 512))
+                sig[bit // 8] ^= 1 << (bit % 8)
+            row[a + 64 * j:a + 64 * (j + 1)] = np.frombuffer(bytes(sig), np.uint8)
+        sizes[p] = size
+        h = m + len(prefix)
+        keys_at = h + 3 + len(short_u16(k))
+        if rng.random() < malformed_frac:
+            bad = MALFORMED_KINDS[int(rng.integers(0, len(MALFORMED_KINDS)))]
+            _break_rule(row, sizes, p, bad, ns, a, m, h, k, keys_at, size, stride, rng)
+            malformed[p] = bad
+            continue
+        base = p * stride
+        sig_lens[p] = ns
+        for j in range(ns):
+            lens.append(size - m)
+            sgo.append(base + a + 64 * j)
+            pko.append(base + keys_at + 32 * j)
+            mso.append(base + m)
+            kinds.append(sig_kind[j])
+    flat = pkts.reshape(-1)
+    u32 = lambda x: np.asarray(x, dtype=np.uint32)
+    return Batch(flat, stride, _split(flat, stride, counts), u32(lens), u32(pko), u32(sgo),
+                 u32(mso), {"sizes": sizes, "sig_lens": sig_lens, "kinds": kinds,
+                            "malformed": malformed, "batch_counts": counts})
+
+
+def _break_rule(row, sizes, p, bad, ns, a, m, h, k, keys_at, size, stride, rng):
+    """Breaks exactly one parsing rule of the packet in row[:size]."""
+    if bad == "short":                       # fewer than 1 + 64 bytes
+        sizes[p] = int(rng.integers(0, 65))
+    elif bad == "oversize":                  # more bytes than the slot holds
+        sizes[p] = stride + int(rng.integers(1, 64))
+    elif bad == "count_alias":               # a non-shortest count: n | 0x80, 0x00
+        row[0] = ns | 0x80
+        row[1] = 0
+    elif bad == "count_too_big":             # signatures would reach past the end
+        row[0] = 0x7F
+    elif bad == "version":                   # message version 1..127
+        row[m] = 0x80 | int(rng.integers(1, 128))
+    elif bad == "payer_readonly":            # readonly signed >= required
+        row[h + 1] = row[h] + int(rng.integers(0, 3))
+    elif bad == "count_mismatch":            # header's required != n
+        row[h] = ns + 1
+        row[h + 1] = 0
+    elif bad == "keys_fewer":                # fewer keys than signatures
+        row[h + 3] = ns - 1
+    elif bad == "keys_overrun":              # the keys run past the end
+        row[h + 3] = 0x7F
+    elif bad == "keys_alias":                # key count 0x80 | k, then 0x00
+        row[h + 3] = 0x80 | (k & 0x7F)
+        row[h + 4] = 0
+    elif bad == "header_cut":                # the packet ends inside the header
+        sizes[p] = h + 3
+    else:
+        raise ValueError(bad)
