@@ -1,11 +1,30 @@
 // Proof-of-History chain verification: hash_i <- SHA-256(hash_i), repeated
-// num_hashes[i] times, one chain per lane.
+// num_hashes[i] times.
 //
 // Reference: src/cuda-poh-verify/poh_verify.cu:12-27 (kernel), :80-144 (ABI).
 // The 32-byte state stays in eight VGPRs as big-endian words for the whole
 // chain: SHA-256 output words are exactly the next message words, so no byte
 // shuffling happens inside the loop, and the padding half of the block is a
 // compile-time constant.
+//
+// Two kernels, by how many chains there are:
+//   poh_chain_kernel  one chain per lane.  Once the chains fill every SIMD
+//                     (64 per wave, one wave per SIMD) this is the fastest
+//                     form: a wave issues its 1,344 instructions per hash
+//                     back to back.
+//   poh_split_kernel  two waves per 64 chains, for fewer chains than that.
+//                     A wave64 VALU instruction occupies its SIMD for 4
+//                     cycles, so one wave per chain group leaves a chain's
+//                     time at 4 cycles x the instructions of one hash, and
+//                     the other SIMDs idle.  Here the second wave of the
+//                     workgroup (on another SIMD) computes the message
+//                     schedule W16..W63 (plus K) of the same hash and hands
+//                     it over through LDS, so the rounds wave issues ~970
+//                     instructions per hash instead of 1,344.  Four barriers
+//                     per hash keep the two waves in step; every one of them
+//                     sits in control flow that both waves run identically
+//                     (the hash count is the workgroup's maximum, and lanes
+//                     past their own count keep their state).
 #include "runtime.h"
 #include "../../include/cuda_crypt.h"
 
@@ -26,13 +45,167 @@ __global__ void __launch_bounds__(256) poh_chain_kernel(uint32_t *__restrict__ h
     for (int k = 0; k < 8; k++) h[k] = bswap32(st[k]);
 }
 
+struct Sha256Vars {
+    uint32_t a, b, c, d, e, f, g, h;
+};
+
+// One SHA-256 round; kw = K[t] + W[t].
+__device__ __forceinline__ void sha256_round(Sha256Vars &s, uint32_t kw) {
+    const uint32_t S1 = xor3_32(rotr32(s.e, 6), rotr32(s.e, 11), rotr32(s.e, 25));
+    const uint32_t ch = s.g ^ (s.e & (s.f ^ s.g));
+    const uint32_t t1 = s.h + S1 + ch + kw;
+    const uint32_t S0 = xor3_32(rotr32(s.a, 2), rotr32(s.a, 13), rotr32(s.a, 22));
+    const uint32_t t2 = S0 + maj32(s.a, s.b, s.c);
+    s.h = s.g; s.g = s.f; s.f = s.e; s.e = s.d + t1;
+    s.d = s.c; s.c = s.b; s.b = s.a; s.a = t1 + t2;
+}
+
+// W[t] for t in [t0, t0 + 16) into the ring w (w[t & 15]), with K[t] + W[t]
+// stored to kw[t - 16][lane].
+template <int T0>
+__device__ __forceinline__ void sha256_schedule16(uint32_t (&w)[16], uint32_t (*kw)[64],
+                                                  uint32_t lane) {
+    const uint32_t K[64] = SPL_SHA256_K;
+#pragma unroll
+    for (int t = T0; t < T0 + 16; t++) {
+        const uint32_t w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
+        const uint32_t s0 = xor3_32(rotr32(w15, 7), rotr32(w15, 18), w15 >> 3);
+        const uint32_t s1 = xor3_32(rotr32(w2, 17), rotr32(w2, 19), w2 >> 10);
+        const uint32_t wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
+        w[t & 15] = wt;
+        kw[t - 16][lane] = wt + K[t];
+    }
+}
+
+constexpr uint32_t kPohSplitThreads = 128;   // wave 0: rounds, wave 1: schedule
+
+__global__ void __launch_bounds__(kPohSplitThreads)
+poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ counts, uint64_t n) {
+    __shared__ uint32_t kw[48][64];              // K[t] + W[t], t = 16..63
+    __shared__ uint32_t msg[8][64];              // message words of the coming hash
+    __shared__ unsigned long long reps_max;
+    const uint32_t K[64] = SPL_SHA256_K;
+    const uint32_t H0[8] = SPL_SHA256_H0;
+    const uint32_t lane = threadIdx.x & 63;
+    const bool rounds = threadIdx.x < 64;        // wave-uniform role
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + lane;
+    const bool valid = i < n;
+    const uint64_t cnt = valid ? counts[i] : 0;
+    if (threadIdx.x == 0) reps_max = 0;
+    __syncthreads();
+    if (rounds) atomicMax(&reps_max, (unsigned long long)cnt);
+    uint32_t st[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) st[k] = rounds && valid ? bswap32(hashes[8 * i + k]) : 0u;
+    if (rounds) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) msg[k][lane] = st[k];
+    }
+    __syncthreads();
+    const uint64_t reps = reps_max;
+    Sha256Vars v{};
+    uint32_t w[16];
+    uint32_t q[16];                  // rounds wave: K + W of the rounds ahead
+    // The rounds wave's phases end at rounds 12, 28, 44 and 64; right after
+    // each barrier it loads K + W for the rest of that phase and the first
+    // four rounds of the next one, so every LDS read has rounds of work in
+    // front of its first use.  The schedule wave writes W16..31, W32..47 and
+    // W48..63 in the first three phases and waits out the fourth.
+    for (uint64_t r = 0; r < reps; r++) {
+        // phase 1: rounds 0..11 from the state | W16..31
+        if (rounds) {
+            v = Sha256Vars{H0[0], H0[1], H0[2], H0[3], H0[4], H0[5], H0[6], H0[7]};
+#pragma unroll
+            for (int t = 0; t < 8; t++) sha256_round(v, st[t] + K[t]);
+            sha256_round(v, 0x80000000u + K[8]);
+#pragma unroll
+            for (int t = 9; t < 12; t++) sha256_round(v, K[t]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) w[k] = msg[k][lane];
+            w[8] = 0x80000000u;
+#pragma unroll
+            for (int k = 9; k < 15; k++) w[k] = 0;
+            w[15] = 256;
+            sha256_schedule16<16>(w, kw, lane);
+        }
+        __syncthreads();
+        // phase 2: rounds 12..27 | W32..47
+        if (rounds) {
+#pragma unroll
+            for (int t = 16; t < 32; t++) q[t & 15] = kw[t - 16][lane];
+            __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the rounds
+#pragma unroll
+            for (int t = 12; t < 15; t++) sha256_round(v, K[t]);
+            sha256_round(v, 256u + K[15]);
+#pragma unroll
+            for (int t = 16; t < 28; t++) sha256_round(v, q[t & 15]);
+        } else {
+            sha256_schedule16<32>(w, kw, lane);
+        }
+        __syncthreads();
+        // phase 3: rounds 28..43 | W48..63
+        if (rounds) {
+            uint32_t p[4];
+#pragma unroll
+            for (int t = 28; t < 32; t++) p[t - 28] = q[t & 15];
+#pragma unroll
+            for (int t = 32; t < 48; t++) q[t & 15] = kw[t - 16][lane];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 28; t < 32; t++) sha256_round(v, p[t - 28]);
+#pragma unroll
+            for (int t = 32; t < 44; t++) sha256_round(v, q[t & 15]);
+        } else {
+            sha256_schedule16<48>(w, kw, lane);
+        }
+        __syncthreads();
+        // phase 4: rounds 44..63, feed-forward, next message
+        if (rounds) {
+            uint32_t p[4];
+#pragma unroll
+            for (int t = 44; t < 48; t++) p[t - 44] = q[t & 15];
+#pragma unroll
+            for (int t = 48; t < 64; t++) q[t & 15] = kw[t - 16][lane];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 44; t < 48; t++) sha256_round(v, p[t - 44]);
+#pragma unroll
+            for (int t = 48; t < 64; t++) sha256_round(v, q[t & 15]);
+            if (r < cnt) {
+                st[0] = H0[0] + v.a; st[1] = H0[1] + v.b; st[2] = H0[2] + v.c;
+                st[3] = H0[3] + v.d; st[4] = H0[4] + v.e; st[5] = H0[5] + v.f;
+                st[6] = H0[6] + v.g; st[7] = H0[7] + v.h;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) msg[k][lane] = st[k];
+        }
+        __syncthreads();
+    }
+    if (rounds && valid) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) hashes[8 * i + k] = bswap32(st[k]);
+    }
+}
+
+// Up to 32 chains per SIMD the split kernel's two waves per 64 chains still
+// give every wave a SIMD of its own.
+static bool use_split_kernel(uint64_t n) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    return n <= 32ull * 4ull * (uint64_t)device_cus(dev);
+}
+
 hipError_t launch_poh(uint32_t *d_hashes, const uint64_t *d_counts, uint64_t n,
                       hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t block = 64;
-    uint64_t grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(poh_chain_kernel, dim3((uint32_t)grid), dim3(block), 0, stream,
-                       d_hashes, d_counts, n);
+    const uint64_t groups = (n + 63) / 64;
+    if (use_split_kernel(n))
+        hipLaunchKernelGGL(poh_split_kernel, dim3((uint32_t)groups), dim3(kPohSplitThreads), 0,
+                           stream, d_hashes, d_counts, n);
+    else
+        hipLaunchKernelGGL(poh_chain_kernel, dim3((uint32_t)groups), dim3(64), 0, stream,
+                           d_hashes, d_counts, n);
     return hipGetLastError();
 }
 

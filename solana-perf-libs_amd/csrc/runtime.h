@@ -106,8 +106,12 @@ void *stream_scratch(int hip_device, hipStream_t stream, size_t bytes);
 // rounded length).  Used instead of DMA copies for host<->device transfers.
 hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 
+// Compute units of a HIP device (256 when the query fails).
+int device_cus(int dev);
+
 // Enqueues SHA-256 chains: hashes[8*i..8*i+8) <- SHA-256^counts[i] of itself,
-// in place, for i < n (poh_verify.hip).
+// in place, for i < n (poh_verify.hip).  Few chains take the two-waves-per-64
+// kernel, many the one-lane-per-chain kernel.
 hipError_t launch_poh(uint32_t *d_hashes, const uint64_t *d_counts, uint64_t n,
                       hipStream_t stream);
 

@@ -80,3 +80,23 @@ def test_device_resident_form(gpu_lib, oracle):
     assert gpu_lib.poh_verify_many_device(dh.ptr, dc.ptr, 3000, s.handle) == 0
     s.synchronize()
     assert np.array_equal(dh.download(), want)
+
+
+def test_both_kernels_vs_oracle(gpu_lib, oracle):
+    """Up to 32 chains per SIMD (32,768 on 256 CUs) run on two waves per 64
+    chains (rounds and message schedule split); more run one chain per lane.
+    Both sides of the switch, with ragged counts including zeros, and a group
+    whose one long chain makes the other 63 idle."""
+    rng = np.random.default_rng(14)
+    for n in (32768, 32769, 40000):
+        h, _ = W.poh_chains(n, 0, seed=n)
+        counts = rng.integers(0, 24, n).astype(np.uint64)
+        counts[rng.random(n) < 0.05] = 0
+        counts[64:128] = 0
+        counts[100] = 300
+        want = h.copy()
+        oracle.poh_verify_many(want, counts, threads=8)
+        got = h.copy()
+        assert gpu_lib.poh_verify_many(got, counts) == 0
+        assert np.array_equal(got, want), n
+        assert np.array_equal(got[32 * 64:32 * 100], h[32 * 64:32 * 100])
