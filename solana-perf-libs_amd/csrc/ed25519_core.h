@@ -79,6 +79,14 @@ struct VarWindow {
     static_assert(W * kWindows - 1 >= 253, "top digit must carry nothing out");
 };
 
+// Keeps the device's instruction scheduler from moving anything across this
+// point (a no-op on the host).
+SPL_HD void sched_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 // ------------------------------------------------------------ message reader
 
 // Little-endian 32-bit word m of "M || 0x80 || 0 ...": the message bytes
@@ -132,12 +140,81 @@ SPL_HD void load_words(uint32_t *out, const uint8_t *src, int n) {
 #endif
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Dwords q0 .. q0+32 of the message's dword array (the message read from its
+// address rounded down to 4), each index clamped into [0, last dword holding
+// a message byte] so every load stays inside the message.  The 33 loads are
+// independent and unconditional (a lane with an empty message loads
+// nothing), so they are all in flight together.
+__device__ __forceinline__ void msg_window(uint32_t d[33], const MsgReader &msg, int q0) {
+    if (msg.len == 0) {
+#pragma unroll
+        for (int k = 0; k < 33; k++) d[k] = 0u;
+        return;
+    }
+    const uintptr_t addr = (uintptr_t)msg.p;
+    const uint32_t *a = (const uint32_t *)(addr & ~(uintptr_t)3);
+    const int qmax = (int)(((uint64_t)(addr & 3) + msg.len - 1u) >> 2);
+#pragma unroll
+    for (int k = 0; k < 33; k++) {
+        int q = q0 + k;
+        q = q < 0 ? 0 : (q > qmax ? qmax : q);
+        d[k] = a[q];
+    }
+}
+
+// Word m of "M || 0x80 || 0 ..." from the window starting at dword q0
+// (0 <= m - q0 < 32): the bytes of the message, the padding byte at position
+// len, zeros after; the same word MsgReader::word returns.
+__device__ __forceinline__ uint32_t msg_window_word(const uint32_t d[33], int i, int m,
+                                                    uint32_t sh, uint32_t len) {
+    const uint32_t w = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+    const int64_t rem = (int64_t)len - 4 * (int64_t)m;
+    const uint32_t keep = rem >= 4 ? ~0u : (rem > 0 ? (1u << (8 * (uint32_t)rem)) - 1u : 0u);
+    const uint32_t term = (rem >= 0 && rem < 4) ? 0x80u << (8 * (uint32_t)rem) : 0u;
+    return (w & keep) | term;
+}
+#endif
+
 // SHA-512 over prefix (npw little-endian words, npw even) || M.
 SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw,
                             const MsgReader &msg) {
     sha512_init(st);
     uint64_t total = 4ull * npw + msg.len;
     uint32_t nblocks = (uint32_t)((total + 17 + 127) / 128);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // Block b holds message words 32 b - npw .. + 31.  Each block's message
+    // dwords are loaded one block ahead, while the previous block compresses,
+    // instead of one dependent load (and wait) per word.
+    const uint32_t sh = (uint32_t)((uintptr_t)msg.p & 3);
+    uint32_t win[33];
+    msg_window(win, msg, -(int)npw);
+    for (uint32_t b = 0; b < nblocks; b++) {
+        const int q0 = 32 * (int)b - (int)npw;
+        uint32_t nxt[33];
+        msg_window(nxt, msg, q0 + 32);
+        sched_fence();
+        uint64_t w[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const uint32_t k = 32u * b + 2u * t;   // stream word index
+            const uint32_t s0 = k < npw ? prefix[k]
+                                        : msg_window_word(win, 2 * t, q0 + 2 * t, sh, msg.len);
+            const uint32_t s1 = k + 1 < npw
+                                    ? prefix[k + 1]
+                                    : msg_window_word(win, 2 * t + 1, q0 + 2 * t + 1, sh, msg.len);
+            w[t] = ((uint64_t)bswap32(s0) << 32) | bswap32(s1);
+        }
+        if (b == nblocks - 1) {
+            w[14] = 0;
+            w[15] = total * 8;
+        }
+        sha512_block(st, w);
+#pragma unroll
+        for (int k = 0; k < 33; k++) win[k] = nxt[k];
+    }
+    return;
+#endif
     for (uint32_t b = 0; b < nblocks; b++) {
         uint64_t w[16];
         for (int t = 0; t < 16; t++) {
@@ -346,7 +423,14 @@ SPL_HD int half_windows(int bits) { return (bits + W) / W; }
 // Stage 2: K windows of W bits (K W <= 256, the same K in every lane of a
 // wave); T1, Tr hold 0..2^(W-1) times P1 and -R.  Returns 1 when
 // [sp]B + [x]P1 + [y](-R) is the identity.
-template <int W>
+//
+// PF: load each window's three table entries (two per-signature tables in
+// global scratch, one comb entry) before its W doublings, which then hide the
+// load latency.  A full batch's per-signature tables are far larger than the
+// L2, so each entry otherwise costs a cache-missing load at its point of use.
+// It holds ~110 more registers live across the doublings, so it suits
+// kernels with a whole SIMD's register file (one wave per SIMD).
+template <int W, bool PF = false>
 SPL_HD int verify_finish_half(const uint32_t x[8], const uint32_t y[8], const uint32_t sp[8],
                               int K, const ge_cached *T1, const ge_cached *Tr,
                               const ed25519_tables &T) {
@@ -365,6 +449,39 @@ SPL_HD int verify_finish_half(const uint32_t x[8], const uint32_t y[8], const ui
     for (int j = K - 1; j >= 0; j--) {
         const int d1 = take_top_digit<W>(a);
         const int d2 = take_top_digit<W>(b);
+        if (PF) {
+            const ge_cached c1 = T1[d1 < 0 ? -d1 : d1];
+            const ge_cached c2 = Tr[d2 < 0 ? -d2 : d2];
+            const bool comb = m < 32;              // the same in every lane
+            int df = 0;
+            ge_precomp cp;
+            if (comb) {
+                df = take_low_byte_digit(s, sbelow);
+                cp = T.comb[m][df < 0 ? -df : df];
+            }
+            sched_fence();
+            if (j != K - 1) {
+#pragma unroll
+                for (int k = 0; k < W - 1; k++) {
+                    ge_p2_dbl(t, r);
+                    ge_p1p1_to_p2(r, t);
+                }
+                ge_p2_dbl(t, r);
+                ge_p1p1_to_p3(u, t);
+            } else {
+                ge_p3_0(u);
+            }
+            if (comb) {
+                ge_add_precomp(tb, pb, cp, df < 0);
+                ge_p1p1_to_p3(pb, tb);
+                m++;
+            }
+            ge_add_cached(t, u, c1, d1 < 0);
+            ge_p1p1_to_p3(u, t);
+            ge_add_cached(t, u, c2, d2 < 0);
+            if (j != 0) ge_p1p1_to_p2(r, t);
+            continue;
+        }
         if (m < 32) {
             const int df = take_low_byte_digit(s, sbelow);
             ge_add_precomp(tb, pb, T.comb[m][df < 0 ? -df : df], df < 0);

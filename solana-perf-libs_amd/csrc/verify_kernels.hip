@@ -36,7 +36,7 @@ __device__ __forceinline__ int wave_max7(int v) {
 template <int W, int HALF>
 SPL_HD constexpr int lane_tables() { return (HALF ? 2 : 1) * VarWindow<W>::kTable; }
 
-template <int W, int HALF>
+template <int W, int HALF, bool PF>
 __device__ __forceinline__ void verify_sig(const VerifyJob &job, uint32_t i, ge_cached *tabs) {
     const uint64_t so = job.sig_off[i], po = job.pk_off[i], mo = job.msg_off[i],
                    ml = job.lens[i];
@@ -65,7 +65,7 @@ __device__ __forceinline__ void verify_sig(const VerifyJob &job, uint32_t i, ge_
     // the same window count in every lane still running keeps the adds at
     // the same bit positions across the wave
     const int K = wave_max7(half_windows<W>(half_bits(x, y)));
-    job.out[i] = (uint8_t)verify_finish_half<W>(x, y, sp, K, tabs, tabs + kT, *job.T);
+    job.out[i] = (uint8_t)verify_finish_half<W, PF>(x, y, sp, K, tabs, tabs + kT, *job.T);
 }
 
 template <int W, int WPE, int HALF>
@@ -75,7 +75,8 @@ verify_plain_kernel(VerifyJob job, uint32_t n_arg) {
     const uint32_t n = job.n_dev ? *job.n_dev : n_arg;
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
     if (i >= n) return;
-    verify_sig<W, HALF>(job, i, job.scratch + (size_t)i * lane_tables<W, HALF>());
+    // one wave per SIMD leaves the registers to prefetch table entries
+    verify_sig<W, HALF, WPE == 1>(job, i, job.scratch + (size_t)i * lane_tables<W, HALF>());
 }
 
 // The value held by lane ^ 1 (ds_swizzle, bit mode: and 0x1f, or 0, xor 1).
@@ -234,7 +235,7 @@ verify_persistent_kernel(VerifyJob job) {
         base = __builtin_amdgcn_readfirstlane(base);
         if (base >= n) break;   // wave-uniform: every wave exits once the queue is drained
         const uint32_t k = base + lane;
-        if (k < n) verify_sig<W, HALF>(job, job.order ? job.order[k] : k, tabs);
+        if (k < n) verify_sig<W, HALF, true>(job, job.order ? job.order[k] : k, tabs);
     }
 }
 
