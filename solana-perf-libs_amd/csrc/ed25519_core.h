@@ -253,20 +253,29 @@ SPL_HD int verify_prepare(const uint32_t sig[16], const uint32_t pk[8], const Ms
     return 1;
 }
 
-// Ai[k] = k A for k = 0..n-1 in cached form (Ai[0] is the identity).
+// Ai[k] = k A for k = 0..n-1 in cached form (Ai[0] is the identity).  Each
+// entry is A2 + the entry two back, which is kept in registers: reading it
+// back from Ai (global scratch on the device) would put a store-to-load round
+// trip through memory on every step.
 SPL_HD void build_multiples(ge_cached *Ai, int n, const ge_p3 &A, const fe &d2) {
     ge_p1p1 t;
     ge_p3 u, A2;
+    ge_cached back2, back1, cur;
     fe_set(Ai[0].YplusX, 1); fe_set(Ai[0].YminusX, 1);
     fe_set(Ai[0].Z, 1); fe_set(Ai[0].T2d, 0);
-    ge_p3_to_cached(Ai[1], A, d2);
+    ge_p3_to_cached(back2, A, d2);
+    Ai[1] = back2;
     ge_p3_dbl(t, A);
     ge_p1p1_to_p3(A2, t);
-    ge_p3_to_cached(Ai[2], A2, d2);
+    ge_p3_to_cached(back1, A2, d2);
+    Ai[2] = back1;
     for (int k = 3; k < n; k++) {
-        ge_add_cached(t, A2, Ai[k - 2], 0);
+        ge_add_cached(t, A2, back2, 0);
         ge_p1p1_to_p3(u, t);
-        ge_p3_to_cached(Ai[k], u, d2);
+        ge_p3_to_cached(cur, u, d2);
+        Ai[k] = cur;
+        back2 = back1;
+        back1 = cur;
     }
 }
 
