@@ -7,6 +7,8 @@
 // buffer; SHA-512's 64-bit words are held as 32-bit halves by the compiler
 // (V_ALIGNBIT_B32 pairs for the rotates, add/addc pairs for the sums).
 #pragma once
+#include <utility>
+
 #include "fe25519.h"
 
 namespace spl {
@@ -19,7 +21,35 @@ SPL_HD uint32_t rotr32(uint32_t x, int n) {
 #endif
 }
 
-SPL_HD uint64_t rotr64(uint64_t x, int n) { return (x >> n) | (x << (64 - n)); }
+// 64-bit rotate and shift on the two 32-bit halves: a rotate is two
+// V_ALIGNBIT_B32 (the compiler otherwise builds it from two 64-bit shifts and
+// two ORs), a right shift by n < 32 one V_ALIGNBIT_B32 and one 32-bit shift.
+// n is a compile-time constant at every use, so the n < 32 test folds.
+SPL_HD uint64_t rotr64(uint64_t x, int n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    uint32_t rl, rh;
+    if (n < 32) {
+        rl = __builtin_amdgcn_alignbit(hi, lo, n);
+        rh = __builtin_amdgcn_alignbit(lo, hi, n);
+    } else {
+        rl = __builtin_amdgcn_alignbit(lo, hi, n - 32);
+        rh = __builtin_amdgcn_alignbit(hi, lo, n - 32);
+    }
+    return ((uint64_t)rh << 32) | rl;
+#else
+    return (x >> n) | (x << (64 - n));
+#endif
+}
+
+SPL_HD uint64_t shr64(uint64_t x, int n) {   // 0 < n < 32
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return ((uint64_t)(hi >> n) << 32) | __builtin_amdgcn_alignbit(hi, lo, n);
+#else
+    return x >> n;
+#endif
+}
 
 // a ^ b ^ c in one V_BITOP3_B32 (truth table 0x96) on gfx950; the compiler
 // otherwise emits two V_XOR_B32 for each of the Sigma/sigma functions.
@@ -160,35 +190,49 @@ SPL_HD void sha512_init(uint64_t st[8]) {
     st[6] = 0x1f83d9abfb41bd6bull; st[7] = 0x5be0cd19137e2179ull;
 }
 
-// One compression; w[0..15] are the block's big-endian 64-bit words and are
-// overwritten (rolling schedule).
-SPL_HD void sha512_block(uint64_t st[8], uint64_t w[16]) {
-    const uint64_t K[80] = SPL_SHA512_K;
-    uint64_t a = st[0], b = st[1], c = st[2], d = st[3];
-    uint64_t e = st[4], f = st[5], g = st[6], h = st[7];
-#pragma unroll
-    for (int t = 0; t < 80; t++) {
-        uint64_t wt;
-        if (t < 16) {
-            wt = w[t];
-        } else {
-            uint64_t w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
-            uint64_t s0 = xor3_64(rotr64(w15, 1), rotr64(w15, 8), w15 >> 7);
-            uint64_t s1 = xor3_64(rotr64(w2, 19), rotr64(w2, 61), w2 >> 6);
-            wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
-            w[t & 15] = wt;
-        }
-        uint64_t S1 = xor3_64(rotr64(e, 14), rotr64(e, 18), rotr64(e, 41));
-        uint64_t ch = g ^ (e & (f ^ g));
-        uint64_t t1 = h + S1 + ch + K[t] + wt;
-        uint64_t S0 = xor3_64(rotr64(a, 28), rotr64(a, 34), rotr64(a, 39));
-        uint64_t maj = maj64(a, b, c);
-        uint64_t t2 = S0 + maj;
-        h = g; g = f; f = e; e = d + t1;
-        d = c; c = b; b = a; a = t1 + t2;
+// Round T of SHA-512 on the working variables v[8], which rotate roles
+// instead of values: a is v[(8 - T mod 8) mod 8], b the next one, and so on.
+// The round writes the new a into h's register and the new e into d's.
+template <int T>
+SPL_HD void sha512_round(uint64_t v[8], uint64_t w[16]) {
+    constexpr uint64_t K[80] = SPL_SHA512_K;
+    constexpr int A = (8 - T % 8) % 8;
+    uint64_t wt;
+    if (T < 16) {
+        wt = w[T];
+    } else {
+        const uint64_t w15 = w[(T - 15) & 15], w2 = w[(T - 2) & 15];
+        const uint64_t s0 = xor3_64(rotr64(w15, 1), rotr64(w15, 8), shr64(w15, 7));
+        const uint64_t s1 = xor3_64(rotr64(w2, 19), rotr64(w2, 61), shr64(w2, 6));
+        wt = w[T & 15] + s0 + w[(T - 7) & 15] + s1;
+        w[T & 15] = wt;
     }
-    st[0] += a; st[1] += b; st[2] += c; st[3] += d;
-    st[4] += e; st[5] += f; st[6] += g; st[7] += h;
+    const uint64_t a = v[A], b = v[(A + 1) % 8], c = v[(A + 2) % 8];
+    const uint64_t e = v[(A + 4) % 8], f = v[(A + 5) % 8], g = v[(A + 6) % 8];
+    const uint64_t S1 = xor3_64(rotr64(e, 14), rotr64(e, 18), rotr64(e, 41));
+    const uint64_t ch = g ^ (e & (f ^ g));
+    const uint64_t t1 = v[(A + 7) % 8] + S1 + ch + K[T] + wt;
+    const uint64_t S0 = xor3_64(rotr64(a, 28), rotr64(a, 34), rotr64(a, 39));
+    v[(A + 3) % 8] += t1;                        // e of the next round
+    v[(A + 7) % 8] = t1 + S0 + maj64(a, b, c);   // a of the next round
+}
+
+template <int... Ts>
+SPL_HD void sha512_rounds(std::integer_sequence<int, Ts...>, uint64_t v[8], uint64_t w[16]) {
+    using expand = int[];
+    (void)expand{0, (sha512_round<Ts>(v, w), 0)...};
+}
+
+// One compression; w[0..15] are the block's big-endian 64-bit words and are
+// overwritten (rolling schedule).  The 80 rounds are straight-line code by
+// construction (no loop for the unroller to weigh).
+SPL_HD void sha512_block(uint64_t st[8], uint64_t w[16]) {
+    uint64_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = st[i];
+    sha512_rounds(std::make_integer_sequence<int, 80>{}, v, w);
+#pragma unroll
+    for (int i = 0; i < 8; i++) st[i] += v[i];
 }
 
 // Digest words (big-endian 64-bit) -> 16 little-endian 32-bit words of the
