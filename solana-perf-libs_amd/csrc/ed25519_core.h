@@ -434,9 +434,11 @@ SPL_HD int half_windows(int bits) { return (bits + W) / W; }
 // [sp]B + [x]P1 + [y](-R) is the identity.
 //
 // PF: load each window's three table entries (two per-signature tables in
-// global scratch, one comb entry) before its W doublings, which then hide the
-// load latency.  A full batch's per-signature tables are far larger than the
-// L2, so each entry otherwise costs a cache-missing load at its point of use.
+// global scratch, far larger than the L2 for a full batch, and one comb
+// entry) before its W doublings, with a scheduling barrier so they stay
+// there and the doublings cover their latency.  Measured on gfx950 the
+// compiler's own schedule already hid most of it (the window loop runs at the
+// single-wave issue rate either way); the barrier makes the order explicit.
 // It holds ~110 more registers live across the doublings, so it suits
 // kernels with a whole SIMD's register file (one wave per SIMD).
 template <int W, bool PF = false>
