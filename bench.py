@@ -129,7 +129,7 @@ def served_leg(lib, b, mask, seconds=1.0):
                    "server": server.stats(reset=True)}
             steps.append(row)
             v = row["verify"]
-            log(f"bench: served {rate:.0e} sig/s offered: {v['units_per_s']:.3e} sig/s, "
+            log(f"bench: served {rate:.1e} sig/s offered: {v['units_per_s']:.3e} sig/s, "
                 f"p50 {v['p50_ms']:.2f} ms, p99 {v['p99_ms']:.2f} ms, "
                 f"poh p99 {row.get('poh', {}).get('p99_ms', 0):.1f} ms, exact "
                 f"{v['all_exact'] and row.get('poh', {'all_exact': True})['all_exact']}")
