@@ -399,9 +399,17 @@ SPL_HD int verify_prepare_half(const uint32_t sig[16], const uint32_t pk[8], con
     uint32_t hw[16], h[8];
     sha512_digest_le_words(hw, st);
     sc_reduce512(h, hw);
-    if (ge_decode_neg(P1, pk, T.k) != 0) return 0;
+    {
+        // -A and -R in one lockstep decode (two independent exponentiations)
+        ge_p3 pts[2];
+        const uint32_t *const enc[2] = {pk, sig};
+        uint32_t bad[2];
+        ge_decode_neg_n(pts, enc, T.k, bad);
+        if (bad[0] | bad[1]) return 0;
+        P1 = pts[0];
+        R = pts[1];
+    }
     if (y_low_order(pk)) return 0;
-    if (ge_decode_neg(R, sig, T.k) != 0) return 0;
     uint32_t x_neg;
     half_scalars(h, x, x_neg, y);
     const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};

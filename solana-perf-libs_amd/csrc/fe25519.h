@@ -29,9 +29,23 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define SPL_HD __host__ __device__ __forceinline__
 
 namespace spl {
+
+// Calls f(std::integral_constant<int, j>{}) for j = 0 .. N-1, expanded at
+// compile time.  Used for the lockstep forms below: a "#pragma unroll" loop
+// whose body holds several field multiplications can be left rolled by the
+// unroller, which puts the per-element arrays in scratch memory.
+template <int N, class F>
+SPL_HD void for_each_n(F &&f) {
+    if constexpr (N > 0) {
+        for_each_n<N - 1>(f);
+        f(std::integral_constant<int, N - 1>{});
+    }
+}
 
 struct fe {
     int32_t v[10];
@@ -288,46 +302,80 @@ SPL_HD uint32_t fe_isneg(const fe &a) {
     return (uint32_t)c.v[0] & 1u;
 }
 
+// The exponent chains below take N independent elements through the same
+// steps in lockstep: every step is applied to all N before the next one, so a
+// loop of squarings holds N independent squarings per iteration and the
+// scheduler can overlap their carry chains (a single chain is one long
+// dependency: each squaring's carries wait for the previous one).
+
+// r[j] <- r[j]^(2^n) for every j.
+template <int N>
+SPL_HD void fe_sqn_n(fe (&r)[N], int n) {
+    for (int i = 0; i < n; i++)
+        for_each_n<N>([&](auto j) { fe_sq(r[j], r[j]); });
+}
+
 // z^(2^250 - 1) and z^11: the common prefix of the inversion and square-root
 // exponent chains.
+template <int N>
+SPL_HD void fe_chain250_n(fe (&z250)[N], fe (&z11)[N], const fe (&z)[N]) {
+    fe z2[N], z9[N], t[N], z5[N], z10[N], z20[N], z50[N], z100[N];
+    for_each_n<N>([&](auto j) { fe_sq(z2[j], z[j]); t[j] = z2[j]; });
+    fe_sqn_n(t, 2);
+    for_each_n<N>([&](auto j) {
+        fe_mul(z9[j], t[j], z[j]);
+        fe_mul(z11[j], z9[j], z2[j]);
+        fe_sq(t[j], z11[j]);
+        fe_mul(z5[j], t[j], z9[j]);
+        t[j] = z5[j];
+    });
+    fe_sqn_n(t, 5);
+    for_each_n<N>([&](auto j) { fe_mul(z10[j], t[j], z5[j]); t[j] = z10[j]; });
+    fe_sqn_n(t, 10);
+    for_each_n<N>([&](auto j) { fe_mul(z20[j], t[j], z10[j]); t[j] = z20[j]; });
+    fe_sqn_n(t, 20);
+    for_each_n<N>([&](auto j) { fe_mul(t[j], t[j], z20[j]); });     // 2^40 - 1
+    fe_sqn_n(t, 10);
+    for_each_n<N>([&](auto j) { fe_mul(z50[j], t[j], z10[j]); t[j] = z50[j]; });
+    fe_sqn_n(t, 50);
+    for_each_n<N>([&](auto j) { fe_mul(z100[j], t[j], z50[j]); t[j] = z100[j]; });
+    fe_sqn_n(t, 100);
+    for_each_n<N>([&](auto j) { fe_mul(t[j], t[j], z100[j]); });    // 2^200 - 1
+    fe_sqn_n(t, 50);
+    for_each_n<N>([&](auto j) { fe_mul(z250[j], t[j], z50[j]); });
+}
+
+// z^((p-5)/8) = z^(2^252 - 3)
+template <int N>
+SPL_HD void fe_pow22523_n(fe (&r)[N], const fe (&z)[N]) {
+    fe z250[N], z11[N];
+    fe_chain250_n(z250, z11, z);
+    fe_sqn_n(z250, 2);
+    for_each_n<N>([&](auto j) { fe_mul(r[j], z250[j], z[j]); });
+}
+
 SPL_HD void fe_chain250(fe &z250, fe &z11, const fe &z) {
-    fe z2, z9, t, z5, z10, z20, z50, z100;
-    fe_sq(z2, z);
-    fe_sqn(t, z2, 2);
-    fe_mul(z9, t, z);
-    fe_mul(z11, z9, z2);
-    fe_sq(t, z11);
-    fe_mul(z5, t, z9);
-    fe_sqn(t, z5, 5);
-    fe_mul(z10, t, z5);
-    fe_sqn(t, z10, 10);
-    fe_mul(z20, t, z10);
-    fe_sqn(t, z20, 20);
-    fe_mul(t, t, z20);        // 2^40 - 1
-    fe_sqn(t, t, 10);
-    fe_mul(z50, t, z10);
-    fe_sqn(t, z50, 50);
-    fe_mul(z100, t, z50);
-    fe_sqn(t, z100, 100);
-    fe_mul(t, t, z100);       // 2^200 - 1
-    fe_sqn(t, t, 50);
-    fe_mul(z250, t, z50);
+    fe a[1], b[1];
+    const fe in[1] = {z};
+    fe_chain250_n(a, b, in);
+    z250 = a[0];
+    z11 = b[0];
 }
 
 // z^(p-2)
 SPL_HD void fe_invert(fe &r, const fe &z) {
-    fe z250, z11;
-    fe_chain250(z250, z11, z);
-    fe_sqn(z250, z250, 5);
-    fe_mul(r, z250, z11);
+    fe z250[1], z11[1];
+    const fe in[1] = {z};
+    fe_chain250_n(z250, z11, in);
+    fe_sqn_n(z250, 5);
+    fe_mul(r, z250[0], z11[0]);
 }
 
-// z^((p-5)/8) = z^(2^252 - 3)
 SPL_HD void fe_pow22523(fe &r, const fe &z) {
-    fe z250, z11;
-    fe_chain250(z250, z11, z);
-    fe_sqn(z250, z250, 2);
-    fe_mul(r, z250, z);
+    fe out[1];
+    const fe in[1] = {z};
+    fe_pow22523_n(out, in);
+    r = out[0];
 }
 
 }  // namespace spl

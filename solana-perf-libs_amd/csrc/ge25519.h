@@ -136,34 +136,58 @@ SPL_HD void ge_encode(uint32_t out[8], const fe &X, const fe &Y, const fe &Z) {
 // x = u v^3 (u v^7)^((p-5)/8) with u = y^2 - 1, v = d y^2 + 1; if v x^2 != u
 // then x *= sqrt(-1) when v x^2 == -u, otherwise the encoding is rejected;
 // finally x is negated when its parity equals bit 255, so x = 0 with bit 255
-// set is accepted.  Returns 0 on success.
+// set is accepted.
+//
+// N encodings are decoded in lockstep (the exponentiation dominates; see
+// fe_chain250_n).  The root selection and the sign fix are branch-free, so
+// the N chains stay in one instruction stream.  bad[j] is 1 when enc[j] is
+// rejected; h[j] is then unspecified.
+template <int N>
+SPL_HD void ge_decode_neg_n(ge_p3 (&h)[N], const uint32_t *const (&enc)[N],
+                            const curve_consts &k, uint32_t (&bad)[N]) {
+    fe u[N], v[N], v3[N], x[N];
+    for_each_n<N>([&](auto j) {
+        fe_from_words(h[j].Y, enc[j]);
+        fe_set(h[j].Z, 1);
+        fe_sq(u[j], h[j].Y);
+        fe_mul(v[j], u[j], k.d);
+        fe_sub(u[j], u[j], h[j].Z);
+        fe_add(v[j], v[j], h[j].Z);
+        fe_sq(v3[j], v[j]);
+        fe_mul(v3[j], v3[j], v[j]);
+        fe_sq(x[j], v3[j]);
+        fe_mul(x[j], x[j], v[j]);
+        fe_mul(x[j], x[j], u[j]);
+    });
+    fe_pow22523_n(x, x);
+    for_each_n<N>([&](auto j) {
+        fe vxx, chk, xi, nx;
+        fe_mul(x[j], x[j], v3[j]);
+        fe_mul(x[j], x[j], u[j]);
+        fe_sq(vxx, x[j]);
+        fe_mul(vxx, vxx, v[j]);
+        fe_sub(chk, vxx, u[j]);
+        const uint32_t root = fe_iszero(chk);          // v x^2 == u
+        fe_add(chk, vxx, u[j]);
+        const uint32_t flip = fe_iszero(chk);          // v x^2 == -u
+        bad[j] = (root | flip) ^ 1u;
+        fe_mul(xi, x[j], k.sqrtm1);
+        fe_cmov(x[j], xi, root ^ 1u);
+        fe_neg(nx, x[j]);
+        fe_cmov(x[j], nx, fe_isneg(x[j]) == (enc[j][7] >> 31));
+        h[j].X = x[j];
+        fe_mul(h[j].T, h[j].X, h[j].Y);
+    });
+}
+
+// One encoding.  Returns 0 on success.
 SPL_HD int ge_decode_neg(ge_p3 &h, const uint32_t enc[8], const curve_consts &k) {
-    fe u, v, v3, vxx, chk;
-    fe_from_words(h.Y, enc);
-    fe_set(h.Z, 1);
-    fe_sq(u, h.Y);
-    fe_mul(v, u, k.d);
-    fe_sub(u, u, h.Z);
-    fe_add(v, v, h.Z);
-    fe_sq(v3, v);
-    fe_mul(v3, v3, v);
-    fe_sq(h.X, v3);
-    fe_mul(h.X, h.X, v);
-    fe_mul(h.X, h.X, u);
-    fe_pow22523(h.X, h.X);
-    fe_mul(h.X, h.X, v3);
-    fe_mul(h.X, h.X, u);
-    fe_sq(vxx, h.X);
-    fe_mul(vxx, vxx, v);
-    fe_sub(chk, vxx, u);
-    if (!fe_iszero(chk)) {
-        fe_add(chk, vxx, u);
-        if (!fe_iszero(chk)) return -1;
-        fe_mul(h.X, h.X, k.sqrtm1);
-    }
-    if (fe_isneg(h.X) == (enc[7] >> 31)) fe_neg(h.X, h.X);
-    fe_mul(h.T, h.X, h.Y);
-    return 0;
+    ge_p3 out[1];
+    const uint32_t *const in[1] = {enc};
+    uint32_t bad[1];
+    ge_decode_neg_n(out, in, k, bad);
+    h = out[0];
+    return bad[0] ? -1 : 0;
 }
 
 // The reference's "small order" decision (ge.cu:60-82 with the raw-limb
