@@ -210,11 +210,6 @@ SPL_HD void fe_sq2(fe &h, const fe &f) {
     fe_reduce_cols(h, c);
 }
 
-SPL_HD void fe_sqn(fe &r, const fe &a, int n) {
-    fe_sq(r, a);
-    for (int i = 1; i < n; i++) fe_sq(r, r);
-}
-
 // Unique representative in [0, p), as limbs in [0, 2^26) / [0, 2^25).
 SPL_HD void fe_canon(fe &r, const fe &a) {
     int32_t h[10];
@@ -352,14 +347,6 @@ SPL_HD void fe_pow22523_n(fe (&r)[N], const fe (&z)[N]) {
     fe_chain250_n(z250, z11, z);
     fe_sqn_n(z250, 2);
     for_each_n<N>([&](auto j) { fe_mul(r[j], z250[j], z[j]); });
-}
-
-SPL_HD void fe_chain250(fe &z250, fe &z11, const fe &z) {
-    fe a[1], b[1];
-    const fe in[1] = {z};
-    fe_chain250_n(a, b, in);
-    z250 = a[0];
-    z11 = b[0];
 }
 
 // z^(p-2)
