@@ -85,19 +85,64 @@ def _torsion_batch(oracle, n=384, seed=5):
     return W.Batch(flat, stride, [flat], lens, base + 64, base.copy(), base + 96)
 
 
+P25519 = 2 ** 255 - 19
+
+
+def _undecodable_batch(oracle, n=320, seed=7):
+    """Signatures whose key, R or both are replaced by random 32-byte strings
+    (about half of them are not on the curve), or by encodings of y = p + k
+    (non-canonical y, accepted by the reference's decoder for keys), mixed in
+    every wave with untouched signatures.  The half-scalar kernels decode A
+    and R together, so this exercises each decode failing alone and both at
+    once."""
+    rng = np.random.default_rng(seed)
+    stride = 256
+    buf = np.zeros((n, stride), np.uint8)
+    for i in range(n):
+        pk, sk = oracle.create_keypair(rng.integers(0, 256, 32, dtype=np.uint8).tobytes())
+        msg = rng.integers(0, 256, int(rng.integers(0, 150)), dtype=np.uint8).tobytes()
+        sig = oracle.sign(msg, pk, sk)
+        kind = i % 6
+        rand = lambda: rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        if kind in (1, 3):
+            pk = rand()
+        if kind in (2, 3):
+            sig = rand() + sig[32:]
+        if kind in (4, 5):
+            y = P25519 + int(rng.integers(2, 19)) + (int(rng.integers(0, 2)) << 255)
+            enc = y.to_bytes(32, "little")
+            if kind == 4:
+                pk = enc
+            else:
+                sig = enc + sig[32:]
+        buf[i, :64] = np.frombuffer(sig, np.uint8)
+        buf[i, 64:96] = np.frombuffer(pk, np.uint8)
+        buf[i, 96:96 + len(msg)] = np.frombuffer(msg, np.uint8)
+        buf[i, 255] = len(msg)
+    base = np.arange(n, dtype=np.uint32) * stride
+    lens = buf[:, 255].astype(np.uint32)
+    flat = buf.reshape(-1)
+    return W.Batch(flat, stride, [flat], lens, base + 64, base.copy(), base + 96)
+
+
 def test_every_configuration(gpu_lib, oracle):
     """Each candidate configuration (one-shot / persistent, W = 4 / 5, full or
-    half-size scalars, register targets, bucketing) gives the oracle's mask."""
+    half-size scalars, register targets, bucketing) gives the oracle's mask on
+    a validator-shaped batch, on torsion-shifted keys and R, and on keys and R
+    that do not decode."""
     import perf_libs
     batches = [
         W.verify_validator(oracle.sign, oracle.create_keypair, seed=23, num_batches=10,
                            num_keys=64, flip_frac=0.1, noncanon_frac=0.03,
                            small_order_frac=0.03),
         _torsion_batch(oracle),
+        _undecodable_batch(oracle),
     ]
     wants = [oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off, threads=8)
              for b in batches]
     assert 0 < wants[1].sum() < wants[1].size
+    # only the untouched rows of the undecodable batch verify
+    assert np.array_equal(np.nonzero(wants[2])[0], np.arange(0, wants[2].size, 6))
     cap = max(b.n for b in batches)
     eng = perf_libs.VerifyEngine(gpu_lib, cap, max(b.packets.size for b in batches))
     try:
