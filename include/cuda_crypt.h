@@ -254,6 +254,17 @@ int spl_verify_packets(const gpu_Elems *elems, uint32_t num_elems, uint32_t pack
 
 #ifdef __cplusplus
 }
+
+/* poh_verify.cu:66-71 — sets up the device pool; true when at least one GPU
+ * is usable.  C++ linkage, as in the reference (the symbol is _Z8poh_initv).
+ * Calling it is optional: every entry point sets the pool up on first use.
+ *
+ * Not exported: get_gpu_ctx, setup_gpu_ctx and release_gpu_ctx
+ * (gpu_ctx.h:12-54).  They hand out the reference's internal pool records,
+ * whose layout holds a cudaStream_t, ge_cached tables and a pthread mutex,
+ * and no caller outside that library uses them.  The pool here is internal;
+ * ed25519_init and ed25519_free_gpu_mem are its public controls. */
+bool poh_init();
 #endif
 
 #endif /* SPL_CUDA_CRYPT_H */

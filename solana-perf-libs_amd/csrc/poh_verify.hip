@@ -213,6 +213,9 @@ hipError_t launch_poh(uint32_t *d_hashes, const uint64_t *d_counts, uint64_t n,
 
 using namespace spl;
 
+// C++ linkage on purpose: the reference exports it as _Z8poh_initv.
+bool poh_init() { return pool_init() > 0; }
+
 extern "C" {
 
 void poh_verify_many_set_verbose(bool val) { g_verbose = val; }

@@ -41,6 +41,9 @@ EXPORTS = (
     "spl_verify_packets",
 )
 
+# Declared with C++ linkage, as the reference exports them: name -> symbol.
+CXX_EXPORTS = {"poh_init": "_Z8poh_initv"}
+
 
 def _p(a, t=_u8p):
     return a.ctypes.data_as(t)
@@ -68,6 +71,9 @@ class PerfLibs:
         lib.poh_verify_many_set_verbose.argtypes = [ctypes.c_bool]
         lib.ed25519_license.restype = ctypes.c_char_p
         lib.ed25519_init.restype = ctypes.c_bool
+        self._poh_init = getattr(lib, CXX_EXPORTS["poh_init"])
+        self._poh_init.argtypes = []
+        self._poh_init.restype = ctypes.c_bool
         lib.cuda_host_register.argtypes = [_vp, ctypes.c_size_t, ctypes.c_uint]
         lib.cuda_host_register.restype = ctypes.c_int
         lib.cuda_host_unregister.argtypes = [_vp]
@@ -133,6 +139,10 @@ class PerfLibs:
     # ------------------------------------------------------------ host API
     def init(self):
         return bool(self.lib.ed25519_init())
+
+    def poh_init(self):
+        """The reference's C++-linkage poh_init() (poh_verify.cu:66-71)."""
+        return bool(self._poh_init())
 
     def license(self):
         return self.lib.ed25519_license().decode()

@@ -10,6 +10,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_reference_fixture(gpu_lib):
+    assert gpu_lib.poh_init() is True
     fx = load_json("poh_reference.json")
     h = np.frombuffer(bytes.fromhex("".join(fx["inputs"])), np.uint8).copy()
     counts = np.array(fx["num_hashes"], np.uint64)

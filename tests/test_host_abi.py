@@ -20,11 +20,12 @@ def test_exports_match_header(lib):
     hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
     declared = set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9 ]+\*?\s*\b([a-z_0-9]+)\(", hdr, re.M))
     declared -= {"if", "return"}
-    assert declared == set(perf_libs.EXPORTS)
+    assert declared == set(perf_libs.EXPORTS) | set(perf_libs.CXX_EXPORTS)
     out = subprocess.run(["nm", "-D", "--defined-only", perf_libs.LIB_PATH],
                          capture_output=True, text=True, check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
-    missing = declared - exported
+    symbols = {perf_libs.CXX_EXPORTS.get(name, name) for name in declared}
+    missing = symbols - exported
     assert not missing, missing
 
 
@@ -102,7 +103,7 @@ def _run_without_gpu(code):
 
 
 def test_no_gpu_behaviour():
-    """Without a device: init is false, poh_verify_many returns 1 and leaves the
+    """Without a device: init and poh_init are false, poh_verify_many returns 1 and leaves the
     hashes alone, verify_many rejects everything instead of crashing
     (the reference dereferences NULL there, verify.cu:265-267), and the engine
     and server constructors refuse."""
@@ -112,6 +113,7 @@ sys.path.insert(0, 'solana-perf-libs_amd')
 import perf_libs
 L = perf_libs.load()
 assert L.init() is False
+assert L.poh_init() is False
 h = np.arange(64, dtype=np.uint8); h0 = h.copy()
 assert L.poh_verify_many(h, np.array([3, 4], np.uint64)) == 1
 assert np.array_equal(h, h0)
