@@ -124,7 +124,7 @@ def served_leg(lib, b, mask, seconds=1.0):
     server = perf_libs.Server(lib, hbm_budget_bytes=1 << 30)
     error = None
     try:
-        for rate in (1e5, 2e5, 5e5, 1e6, 2e6, 4e6, 8e6, 12e6, 16e6):
+        for rate in (1e5, 2e5, 5e5, 1e6, 2e6, 4e6, 8e6, 12e6, 14e6, 16e6, 20e6, 24e6):
             trace = loadgen.make_trace(pools, rng, rate, seconds)
             try:
                 recs, span, _ = loadgen.run_served(server, trace)

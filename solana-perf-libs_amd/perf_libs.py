@@ -342,30 +342,45 @@ class Server:
         self._live = {}
         self._times = (ctypes.c_uint64 * 3)()
 
-    def submit_verify(self, packets, lens, pk_off, sig_off, msg_off, out=None):
-        """One request in the ed25519_verify_many layout (offsets relative to
-        `packets`).  Returns (ticket, out); out is filled on completion."""
+    def prepare_verify(self, packets, lens, pk_off, sig_off, msg_off, out=None):
+        """The arguments of one verify request (ed25519_verify_many layout,
+        offsets relative to `packets`), converted for the C call ahead of
+        time.  Returns (prepared, out); submit with submit_prepared()."""
         tabs = [np.ascontiguousarray(t, dtype=np.uint32) for t in (lens, pk_off, sig_off, msg_off)]
         packets = np.ascontiguousarray(packets, dtype=np.uint8)
         n = tabs[0].size
         if out is None:
             out = np.zeros(max(n, 1), np.uint8)
-        t = self.lib.spl_server_submit_verify(self.h, packets.ctypes.data, packets.size,
-                                              *[x.ctypes.data for x in tabs], n, out.ctypes.data)
-        if t > 0:
-            self._live[t] = (packets, tabs, out)
-        return t, out
+        args = (packets.ctypes.data, packets.size, *[x.ctypes.data for x in tabs], n,
+                out.ctypes.data)
+        return (self.lib.spl_server_submit_verify, args, (packets, tabs, out)), out
 
-    def submit_poh(self, hashes, counts):
+    def prepare_poh(self, hashes, counts):
         """hashes: uint8 [32*n], hashed in place on completion."""
         counts = np.ascontiguousarray(counts, dtype=np.uint64)
         assert hashes.dtype == np.uint8 and hashes.flags.c_contiguous
         assert hashes.size == 32 * counts.size
-        t = self.lib.spl_server_submit_poh(self.h, hashes.ctypes.data, counts.ctypes.data,
-                                           counts.size)
+        args = (hashes.ctypes.data, counts.ctypes.data, counts.size)
+        return (self.lib.spl_server_submit_poh, args, (hashes, counts))
+
+    def submit_prepared(self, prepared):
+        """Submits a request from prepare_verify / prepare_poh; returns the
+        ticket (> 0) or a refusal code."""
+        fn, args, keep = prepared
+        t = fn(self.h, *args)
         if t > 0:
-            self._live[t] = (hashes, counts)
+            self._live[t] = keep
         return t
+
+    def submit_verify(self, packets, lens, pk_off, sig_off, msg_off, out=None):
+        """One request in the ed25519_verify_many layout (offsets relative to
+        `packets`).  Returns (ticket, out); out is filled on completion."""
+        prepared, out = self.prepare_verify(packets, lens, pk_off, sig_off, msg_off, out)
+        return self.submit_prepared(prepared), out
+
+    def submit_poh(self, hashes, counts):
+        """hashes: uint8 [32*n], hashed in place on completion."""
+        return self.submit_prepared(self.prepare_poh(hashes, counts))
 
     def wait(self, ticket, timeout_ms=-1):
         """Returns (rc, (t_submit, t_admit, t_done) in CLOCK_MONOTONIC ns)."""
