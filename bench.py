@@ -21,6 +21,7 @@ import json
 import math
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -108,10 +109,38 @@ def run_cpu_baseline(b, seconds=10.0):
                       f"{src}), {threads} threads, {dt:.1f} s"}, first
 
 
-def served_leg(lib, b, mask, seconds=1.0):
+class Watchdog:
+    """Ends the run with the result line gathered so far when the leg it
+    watches makes no progress for limit_s seconds.  The serving loop went
+    silent twice on the GPU with no cause found; a stall there must not cost
+    the run its measured result."""
+
+    def __init__(self, result, key, limit_s):
+        self.result, self.key, self.limit_s = result, key, limit_s
+        self.last = time.monotonic()
+        self.done = threading.Event()
+        threading.Thread(target=self._watch, daemon=True).start()
+
+    def kick(self):
+        self.last = time.monotonic()
+
+    def close(self):
+        self.done.set()
+
+    def _watch(self):
+        while not self.done.wait(1.0):
+            if time.monotonic() - self.last > self.limit_s:
+                msg = f"no progress for {self.limit_s:.0f} s; leg abandoned"
+                log(f"bench: {self.key}: {msg}")
+                self.result[self.key] = {"error": msg}
+                print(json.dumps(self.result), flush=True)
+                os._exit(0)
+
+
+def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None):
     """Served mixed load: Poisson verify + PoH requests through spl_server at
     stepped offered loads, beside back-to-back plain ed25519_verify_many calls
-    on the same requests."""
+    on the same requests.  progress() is called after each step."""
     import loadgen
     import perf_libs
     import workloads as W
@@ -121,7 +150,9 @@ def served_leg(lib, b, mask, seconds=1.0):
     pools = loadgen.RequestPools(lib, b, mask, poh_in, counts, poh_out)
     rng = np.random.default_rng(12)
     steps = []
+    log("bench: served leg: creating the server")
     server = perf_libs.Server(lib, hbm_budget_bytes=1 << 30)
+    progress()
     error = None
     try:
         for rate in (1e5, 2e5, 5e5, 1e6, 2e6, 4e6, 8e6, 12e6, 14e6, 16e6, 20e6, 24e6):
@@ -136,6 +167,7 @@ def served_leg(lib, b, mask, seconds=1.0):
             row = {"offered_sigs_per_s": rate, **loadgen.summarize(recs, span),
                    "server": server.stats(reset=True)}
             steps.append(row)
+            progress()
             v = row["verify"]
             log(f"bench: served {rate:.1e} sig/s offered: {v['units_per_s']:.3e} sig/s, "
                 f"p50 {v['p50_ms']:.2f} ms, p99 {v['p99_ms']:.2f} ms, "
@@ -143,6 +175,7 @@ def served_leg(lib, b, mask, seconds=1.0):
                 f"{v['all_exact'] and row.get('poh', {'all_exact': True})['all_exact']}")
     finally:
         server.close()
+    progress()
     trace = loadgen.make_trace(pools, np.random.default_rng(13), 1e6, 0.5)
     b2b = loadgen.run_back_to_back(lib, trace)
     log(f"bench: back-to-back plain calls: {b2b['sig_verifies_per_s']:.3e} sig/s, "
@@ -520,9 +553,6 @@ def main():
     if not args.no_extra:
         result["packets"] = packets_leg(lib, stream)
 
-    if not args.no_extra and not args.no_serve:
-        result["served"] = served_leg(lib, b, mask)
-
     if not args.no_cpu:
         log("bench: CPU baseline")
         cb, cpu_mask = run_cpu_baseline(b)
@@ -531,6 +561,12 @@ def main():
         if not args.no_extra:
             result["cpu_baseline_poh_sign"] = run_cpu_extra_baselines()
             result["cpu_poh_multibuffer"] = run_cpu_multibuffer_poh()
+
+    # last, under a watchdog: every other figure is already in the result
+    if not args.no_extra and not args.no_serve:
+        dog = Watchdog(result, "served", limit_s=150.0)
+        result["served"] = served_leg(lib, b, mask, progress=dog.kick)
+        dog.close()
     print(json.dumps(result), flush=True)
 
 

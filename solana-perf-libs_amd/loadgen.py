@@ -11,6 +11,7 @@ requests are runs of chains of one chain pool.  A verify request carries its
 own packet bytes (a view into the pool) and offset tables rebased to them, as
 a caller of ed25519_verify_many would build them.
 """
+import gc
 import time
 
 import numpy as np
@@ -127,15 +128,24 @@ def run_served(server, trace, lead_s=0.05, wait_ms=60000):
         prepared.append((arr, kind, req, prep, res))
     t0 = time.monotonic_ns() + int(lead_s * 1e9)
     tickets = []
-    for arr, kind, req, prep, res in prepared:
-        target = t0 + int(arr * 1e9)
-        while True:
-            now = time.monotonic_ns()
-            if now >= target:
-                break
-            if target - now > 300_000:
-                time.sleep((target - now - 200_000) / 1e9)
-        tickets.append((server.submit_prepared(prep), target, kind, req, res))
+    # No cyclic garbage collection while submitting: a full collection over
+    # tens of thousands of live request objects pauses this thread for
+    # milliseconds, and every request due meanwhile would be submitted late.
+    gc_was_enabled = gc.isenabled()
+    gc.disable()
+    try:
+        for arr, kind, req, prep, res in prepared:
+            target = t0 + int(arr * 1e9)
+            while True:
+                now = time.monotonic_ns()
+                if now >= target:
+                    break
+                if target - now > 300_000:
+                    time.sleep((target - now - 200_000) / 1e9)
+            tickets.append((server.submit_prepared(prep), target, kind, req, res))
+    finally:
+        if gc_was_enabled:
+            gc.enable()
     t_last = time.monotonic_ns()
     recs = []
     for t, target, kind, req, res in tickets:

@@ -14,6 +14,13 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
 
 
+def pytest_collection_modifyitems(session, config, items):
+    """The serving-loop tests run last: each one starts a loop thread and
+    hardware queues of its own, and a stall there must not keep the kernel
+    parity tests from running."""
+    items.sort(key=lambda it: it.nodeid.split("::")[0].endswith("test_gpu_server.py"))
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import Oracle
