@@ -52,7 +52,7 @@ struct Sha256Vars {
 // One SHA-256 round; kw = K[t] + W[t].
 __device__ __forceinline__ void sha256_round(Sha256Vars &s, uint32_t kw) {
     const uint32_t S1 = xor3_32(rotr32(s.e, 6), rotr32(s.e, 11), rotr32(s.e, 25));
-    const uint32_t ch = s.g ^ (s.e & (s.f ^ s.g));
+    const uint32_t ch = ch32(s.e, s.f, s.g);
     const uint32_t t1 = s.h + S1 + ch + kw;
     const uint32_t S0 = xor3_32(rotr32(s.a, 2), rotr32(s.a, 13), rotr32(s.a, 22));
     const uint32_t t2 = S0 + maj32(s.a, s.b, s.c);
@@ -105,7 +105,9 @@ poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ cou
     const uint64_t reps = reps_max;
     Sha256Vars v{};
     uint32_t w[16];
-    uint32_t q[16];                  // rounds wave: K + W of the rounds ahead
+    // rounds wave: K + W of t = 16..31, 32..47 and 48..63, one array per
+    // phase so the four rounds carried into the next phase need no copies
+    uint32_t qa[16], qb[16], qc[16];
     // The rounds wave's phases end at rounds 12, 28, 44 and 64; right after
     // each barrier it loads K + W for the rest of that phase and the first
     // four rounds of the next one, so every LDS read has rounds of work in
@@ -133,45 +135,39 @@ poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ cou
         // phase 2: rounds 12..27 | W32..47
         if (rounds) {
 #pragma unroll
-            for (int t = 16; t < 32; t++) q[t & 15] = kw[t - 16][lane];
+            for (int t = 16; t < 32; t++) qa[t - 16] = kw[t - 16][lane];
             __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the rounds
 #pragma unroll
             for (int t = 12; t < 15; t++) sha256_round(v, K[t]);
             sha256_round(v, 256u + K[15]);
 #pragma unroll
-            for (int t = 16; t < 28; t++) sha256_round(v, q[t & 15]);
+            for (int t = 16; t < 28; t++) sha256_round(v, qa[t - 16]);
         } else {
             sha256_schedule16<32>(w, kw, lane);
         }
         __syncthreads();
         // phase 3: rounds 28..43 | W48..63
         if (rounds) {
-            uint32_t p[4];
 #pragma unroll
-            for (int t = 28; t < 32; t++) p[t - 28] = q[t & 15];
-#pragma unroll
-            for (int t = 32; t < 48; t++) q[t & 15] = kw[t - 16][lane];
+            for (int t = 32; t < 48; t++) qb[t - 32] = kw[t - 16][lane];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int t = 28; t < 32; t++) sha256_round(v, p[t - 28]);
+            for (int t = 28; t < 32; t++) sha256_round(v, qa[t - 16]);
 #pragma unroll
-            for (int t = 32; t < 44; t++) sha256_round(v, q[t & 15]);
+            for (int t = 32; t < 44; t++) sha256_round(v, qb[t - 32]);
         } else {
             sha256_schedule16<48>(w, kw, lane);
         }
         __syncthreads();
         // phase 4: rounds 44..63, feed-forward, next message
         if (rounds) {
-            uint32_t p[4];
 #pragma unroll
-            for (int t = 44; t < 48; t++) p[t - 44] = q[t & 15];
-#pragma unroll
-            for (int t = 48; t < 64; t++) q[t & 15] = kw[t - 16][lane];
+            for (int t = 48; t < 64; t++) qc[t - 48] = kw[t - 16][lane];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int t = 44; t < 48; t++) sha256_round(v, p[t - 44]);
+            for (int t = 44; t < 48; t++) sha256_round(v, qb[t - 32]);
 #pragma unroll
-            for (int t = 48; t < 64; t++) sha256_round(v, q[t & 15]);
+            for (int t = 48; t < 64; t++) sha256_round(v, qc[t - 48]);
             if (r < cnt) {
                 st[0] = H0[0] + v.a; st[1] = H0[1] + v.b; st[2] = H0[2] + v.c;
                 st[3] = H0[3] + v.d; st[4] = H0[4] + v.e; st[5] = H0[5] + v.f;

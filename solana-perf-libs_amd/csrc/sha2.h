@@ -72,6 +72,19 @@ SPL_HD uint32_t maj32(uint32_t a, uint32_t b, uint32_t c) {
 #endif
 }
 
+// Choose: e ? f : g, bit by bit (truth table 0xca; V_BITOP3_B32 indexes its
+// table by (S0, S1, S2) with S0 as the high bit).  Written as
+// g ^ (e & (f ^ g)) the compiler forms one instruction in the one-lane PoH
+// and SHA-512 rounds but two (V_XOR_B32 + V_BITOP3_B32) in the split PoH
+// kernel's rounds.
+SPL_HD uint32_t ch32(uint32_t e, uint32_t f, uint32_t g) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(e, f, g, 0xca);
+#else
+    return g ^ (e & (f ^ g));
+#endif
+}
+
 SPL_HD uint64_t maj64(uint64_t a, uint64_t b, uint64_t c) {
     return ((uint64_t)maj32((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32))
             << 32) |

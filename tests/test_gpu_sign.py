@@ -33,3 +33,13 @@ def test_sign_many_edges(gpu_lib, oracle):
     lens = [len(r[2]) for r in rows]
     got = gpu_lib.sign_many([buf.reshape(-1)], stride, lens, base + 3, base + 37, base + 105)
     assert [got[64 * i:64 * i + 64].tobytes() for i in range(len(rows))] == [r[3] for r in rows]
+
+
+def test_sign_many_full_batch(gpu_lib, oracle):
+    """The full batch-sign workload: 65,536 seeded keypairs, messages of
+    32-512 B, every signature bit-identical to the oracle signer's."""
+    b = W.sign_batch(65536, oracle.create_keypair, seed=4)
+    print(f"sign_many: {b.n} signatures, messages {b.lens.min()}-{b.lens.max()} B", flush=True)
+    got = gpu_lib.sign_many(b.batches, b.message_size, b.lens, b.pk_off, b.sig_off, b.msg_off)
+    want = oracle.sign_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off)
+    assert np.array_equal(got, want)

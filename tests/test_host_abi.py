@@ -134,3 +134,34 @@ print('ok')
 """
     r = _run_without_gpu(code)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
+
+
+def test_check_helpers_full_mix(lib, oracle):
+    """Both host check helpers over 100k random 32-byte encodings, 1 % of them
+    replaced by small-order encodings (canonical and non-canonical) and 1 %
+    by non-reduced scalars (top byte 0x10-0x7F), with zeroed out buffers:
+    every return code and output equals the oracle's."""
+    rng = np.random.default_rng(6)
+    n = 100_000
+    encs = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+    pick = rng.random(n)
+    small = np.nonzero(pick < 0.01)[0]
+    for k, i in enumerate(small):
+        encs[i] = np.frombuffer(W.SMALL_ORDER_KEYS[k % len(W.SMALL_ORDER_KEYS)], np.uint8)
+    unreduced = np.nonzero((pick >= 0.01) & (pick < 0.02))[0]
+    encs[unreduced, 31] = 0x10 | (encs[unreduced, 31] & 0x6f)
+    so_rejects = gc_rejects = 0
+    for i in range(n):
+        e = encs[i].tobytes()
+        so = lib.check_packed_ge_small_order(e)
+        assert so == oracle.check_packed_ge_small_order(e), i
+        gc = lib.get_checked_scalar(e)
+        assert gc == oracle.get_checked_scalar(e), i
+        so_rejects += so
+        gc_rejects += gc[0]
+    # every planted case is rejected: small-order ones by the closed-form
+    # rule (decodes, y mod p in {0, 1, p-1}), non-reduced scalars outright
+    planted = sum(E.small_order_rule(W.SMALL_ORDER_KEYS[k % len(W.SMALL_ORDER_KEYS)])
+                  for k in range(small.size))
+    assert planted > 0 and so_rejects >= planted
+    assert gc_rejects >= unreduced.size
