@@ -50,10 +50,19 @@ struct Sha256Vars {
 };
 
 // One SHA-256 round; kw = K[t] + W[t].
+//
+// h + kw is known rounds ahead (h is e of three rounds back), so it is summed
+// first and kept as one value: the empty asm stops the compiler from
+// reassociating it into (h + S1) + ch + kw, which put one more dependent add
+// on the e chain (now e -> rotates -> xor3 -> add3 -> add).  Measured on the
+// split kernel: 0.4-0.7 % faster at 512-32,768 chains; the rounds wave is
+// bound by a lone wave's issue rate, not by this chain.
 __device__ __forceinline__ void sha256_round(Sha256Vars &s, uint32_t kw) {
     const uint32_t S1 = xor3_32(rotr32(s.e, 6), rotr32(s.e, 11), rotr32(s.e, 25));
     const uint32_t ch = ch32(s.e, s.f, s.g);
-    const uint32_t t1 = s.h + S1 + ch + kw;
+    uint32_t u = s.h + kw;
+    asm("" : "+v"(u));
+    const uint32_t t1 = u + S1 + ch;
     const uint32_t S0 = xor3_32(rotr32(s.a, 2), rotr32(s.a, 13), rotr32(s.a, 22));
     const uint32_t t2 = S0 + maj32(s.a, s.b, s.c);
     s.h = s.g; s.g = s.f; s.f = s.e; s.e = s.d + t1;
