@@ -93,7 +93,7 @@ def _wait_ok(server, t):
 
 
 @pytest.mark.skip(reason="hung once on the GPU (silent >180 s) with no cause found by "
-                         "reading the code; kept off the GPU for this round")
+                         "reading the code; kept off the GPU until a cause is found")
 def test_alone_equals_mix(gpu_lib, pools):
     reqs = _requests(pools, seed=1, n_verify=48, n_poh=10)
     server = _server(gpu_lib, hbm_budget_bytes=64 << 20)
@@ -172,7 +172,7 @@ def test_small_ring_backpressure(gpu_lib, pools):
 
 
 @pytest.mark.skip(reason="went silent for >180 s on the GPU (the run was killed) with no cause "
-                         "found by reading the code; kept off the GPU for this round")
+                         "found by reading the code; kept off the GPU until a cause is found")
 def test_edges(gpu_lib, pools):
     server = _server(gpu_lib, hbm_budget_bytes=8 << 20)
     try:
