@@ -67,20 +67,18 @@ hipError_t stage_packets(QueueCtx &q, const gpu_Elems *elems, uint32_t num_elems
     if ((e = q.h_stage.reserve(tab_at + tab_bytes + 64 * (uint64_t)n + 16)) != hipSuccess)
         return e;
     uint8_t *h = q.h_stage.as<uint8_t>();
-    uint64_t cur = 0;
-    for (uint32_t k = 0; k < num_elems; k++) {
-        uint64_t b = (uint64_t)elems[k].num * message_size;
-        if (cur + b > packets_bytes) b = packets_bytes - cur;   // never overrun
-        if (b) memcpy(h + cur, elems[k].elems, b);
-        cur += b;
-    }
-    if (cur < tab_at) memset(h + cur, 0, tab_at - cur);
+    uint8_t *d = q.d_packets.as<uint8_t>();
+    uint64_t sent = 0;
+    if ((e = stage_elems(h, d, elems, num_elems, message_size, packets_bytes, &sent, s)) !=
+        hipSuccess)
+        return e;
+    memset(h + packets_bytes, 0, tab_at - packets_bytes);   // pad to the tables
     uint32_t *tab = reinterpret_cast<uint32_t *>(h + tab_at);
     memcpy(tab, t0, 4ull * n);
     memcpy(tab + n, t1, 4ull * n);
     memcpy(tab + 2ull * n, t2, 4ull * n);
     memcpy(tab + 3ull * n, t3, 4ull * n);
-    if ((e = launch_copy16(q.d_packets.ptr, h, packets_bytes, s)) != hipSuccess) return e;
+    if ((e = launch_copy16(d + sent, h + sent, packets_bytes - sent, s)) != hipSuccess) return e;
     return launch_copy16(q.d_offsets.ptr, tab, tab_bytes, s);
 }
 

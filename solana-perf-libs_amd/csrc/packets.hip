@@ -411,23 +411,18 @@ int spl_verify_packets(const gpu_Elems *elems, uint32_t num_elems, uint32_t pack
     const uint64_t lens_at = (ok_at + total_packets + 15) & ~15ull;
     const VerifyPacketsScratch v = verify_packets_scratch(total_packets, packet_stride, cus);
     uint8_t *h = nullptr, *d_in = nullptr, *d_res = nullptr;
-    uint64_t cur = 0;
+    uint64_t sent = 0;
     SPL_HIP_TRY(q.h_stage.reserve(lens_at + 4ull * total_packets + 16));
     SPL_HIP_TRY(q.d_packets.reserve(in_bytes + 16));
     SPL_HIP_TRY(q.d_out.reserve(lens_at - ok_at + 4ull * total_packets + 16));
     SPL_HIP_TRY(q.d_scratch.reserve(v.bytes));
     h = q.h_stage.as<uint8_t>();
-    for (uint32_t k = 0; k < num_elems; k++) {
-        uint64_t b = (uint64_t)elems[k].num * packet_stride;
-        if (cur + b > pbytes) b = pbytes - cur;   // never overrun
-        if (b) memcpy(h + cur, elems[k].elems, b);
-        cur += b;
-    }
-    if (cur < sizes_at) memset(h + cur, 0, sizes_at - cur);
-    memcpy(h + sizes_at, packet_sizes, 4ull * total_packets);
     d_in = q.d_packets.as<uint8_t>();
     d_res = q.d_out.as<uint8_t>();
-    SPL_HIP_TRY(launch_copy16(d_in, h, in_bytes, s));
+    SPL_HIP_TRY(stage_elems(h, d_in, elems, num_elems, packet_stride, pbytes, &sent, s));
+    memset(h + pbytes, 0, sizes_at - pbytes);
+    memcpy(h + sizes_at, packet_sizes, 4ull * total_packets);
+    SPL_HIP_TRY(launch_copy16(d_in + sent, h + sent, in_bytes - sent, s));
     SPL_HIP_TRY(enqueue_verify_packets(l.dev->d_tables, cus, d_in, total_packets, packet_stride,
                                        reinterpret_cast<const uint32_t *>(d_in + sizes_at), d_res,
                                        reinterpret_cast<uint32_t *>(d_res + (lens_at - ok_at)),

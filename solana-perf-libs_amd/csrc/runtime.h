@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "ed25519_core.h"
+#include "../../include/cuda_crypt.h"
 
 namespace spl {
 
@@ -105,6 +106,19 @@ void *stream_scratch(int hip_device, hipStream_t stream, size_t bytes);
 // (device or pinned host memory; both 16-byte aligned, both valid for the
 // rounded length).  Used instead of DMA copies for host<->device transfers.
 hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t s);
+
+// Copies the packets of elems[0..num_elems) (elems[k].num packets of
+// `stride` bytes each, concatenated, cut at `total` bytes) into pinned host
+// memory h, and sends them on to device memory d with copy kernels on s as
+// they land: each time another kStageChunk bytes are in h, that chunk's copy
+// kernel is enqueued, so the PCIe transfer of one chunk overlaps the host copy
+// of the next.  On return h[0, total) is written (zeros after the elems'
+// bytes if they hold fewer), and the first *sent bytes (a multiple of 16) are
+// enqueued for d; the caller fills what follows in h and copies [*sent, end)
+// itself.
+constexpr uint64_t kStageChunk = 4ull << 20;
+hipError_t stage_elems(uint8_t *h, uint8_t *d, const gpu_Elems *elems, uint32_t num_elems,
+                       uint32_t stride, uint64_t total, uint64_t *sent, hipStream_t s);
 
 // Compute units of a HIP device (256 when the query fails).
 int device_cus(int dev);

@@ -33,6 +33,35 @@ hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t
     return hipGetLastError();
 }
 
+hipError_t stage_elems(uint8_t *h, uint8_t *d, const gpu_Elems *elems, uint32_t num_elems,
+                       uint32_t stride, uint64_t total, uint64_t *sent, hipStream_t s) {
+    uint64_t cur = 0, done = 0;
+    for (uint32_t k = 0; k < num_elems && cur < total; k++) {
+        uint64_t b = (uint64_t)elems[k].num * stride;
+        if (cur + b > total) b = total - cur;   // never overrun
+        const uint8_t *src = elems[k].elems;
+        while (b) {
+            // fill up to the next chunk boundary, then send every whole chunk
+            const uint64_t room = kStageChunk - (cur - done) % kStageChunk;
+            const uint64_t m = b < room ? b : room;
+            memcpy(h + cur, src, m);
+            cur += m;
+            src += m;
+            b -= m;
+            if (cur - done >= kStageChunk) {
+                const uint64_t len = (cur - done) & ~15ull;
+                const hipError_t e = launch_copy16(d + done, h + done, len, s);
+                if (e != hipSuccess) return e;
+                done += len;
+            }
+        }
+    }
+    // fewer packets in elems than total_packets says: the rest reads as zeros
+    if (cur < total) memset(h + cur, 0, total - cur);
+    *sent = done;
+    return hipSuccess;
+}
+
 hipError_t Buf::reserve(size_t bytes) {
     if (bytes <= cap && ptr) return hipSuccess;
     release();

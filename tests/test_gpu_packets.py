@@ -96,6 +96,23 @@ def test_offsets_fuzzed_headers(gpu_lib, oracle):
         assert np.array_equal(got, want)
 
 
+def test_host_abi_staging_past_one_chunk(gpu_lib, oracle):
+    """The host entry points stage packets through pinned memory in 4 MiB
+    chunks, each sent to the device as soon as it is full.  Here the packets
+    span several chunks and 60 elems, so chunk boundaries fall inside elems:
+    verdicts and the per-signature mask still equal the oracle's."""
+    b = _wire(oracle, seed=35, num_batches=60, max_packets=256, malformed_frac=0.05)
+    print(f"staging: {b.packets.size} bytes in {len(b.batches)} elems", flush=True)
+    assert b.packets.size > 2 * (4 << 20)
+    sizes = b.meta["sizes"]
+    want_lens, want_tabs, mask, want_ok = _expected(oracle, b.packets, b.message_size, sizes)
+    rc, ok, lens = gpu_lib.verify_packets(b.batches, b.message_size, sizes)
+    assert rc == 0
+    assert np.array_equal(lens, want_lens)
+    assert np.array_equal(ok, want_ok)
+    assert np.array_equal(gpu_lib.verify_many(b.batches, b.message_size, *want_tabs), mask)
+
+
 def test_verify_packets_host_abi(gpu_lib, oracle):
     b = _wire(oracle, seed=34, num_batches=24, max_packets=128)
     sizes = b.meta["sizes"]
