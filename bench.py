@@ -188,6 +188,38 @@ def in_flight_leg(lib, b, eng, mask, steps, depths=(2, 3)):
     return out
 
 
+def abi_callers_leg(lib, b, mask, calls=8, thread_counts=(1, 2, 4)):
+    """The validator batch through the drop-in ed25519_verify_many from host
+    buffers (staging and PCIe included), called from T threads at once, as
+    the reference's callers do (any thread may call; the library hands each
+    call a queue of its own).  Whole-job sig-verifies/s over T x `calls`
+    calls; every returned mask checked."""
+    import threading
+    out = {}
+    args = (b.batches, b.message_size, b.lens, b.pk_off, b.sig_off, b.msg_off)
+    for _ in range(8):                          # grow and touch every queue's buffers
+        lib.verify_many(*args)
+    for t in thread_counts:
+        ok = [True] * t
+
+        def caller(k):
+            for _ in range(calls):
+                ok[k] = ok[k] and bool(np.array_equal(lib.verify_many(*args), mask))
+
+        ths = [threading.Thread(target=caller, args=(k,)) for k in range(t)]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        wall = time.perf_counter() - t0
+        out[f"threads_{t}"] = {"sig_verifies_per_s": t * calls * b.n / wall,
+                               "ms_per_call": wall * 1e3 / calls, "masks_exact": all(ok)}
+        log(f"bench: ed25519_verify_many from {t} threads: {t * calls * b.n / wall:.3e} sig/s, "
+            f"exact {all(ok)}")
+    return out
+
+
 def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None):
     """Served mixed load: Poisson verify + PoH requests through spl_server at
     stepped offered loads, beside back-to-back plain ed25519_verify_many calls
@@ -610,6 +642,7 @@ def main():
 
     if not args.no_extra:
         result["packets"] = packets_leg(lib, stream)
+        result["abi_callers"] = abi_callers_leg(lib, b, mask)
 
     if not args.no_cpu:
         log("bench: CPU baseline")

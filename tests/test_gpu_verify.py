@@ -138,3 +138,58 @@ def test_device_resident_form(gpu_lib, oracle):
     assert rc == 0
     s.synchronize()
     assert np.array_equal(dout.download(), want)
+
+
+def test_concurrent_callers(gpu_lib, oracle):
+    """Any thread may call the ABI concurrently (the reference serialises per
+    context, gpu_ctx.cu:46-73): 8 threads, each with its own batch shape, call
+    ed25519_verify_many, ed25519_sign_many and poh_verify_many at once, three
+    times each, and every result equals the oracle's."""
+    import threading
+    jobs = []
+    for t in range(8):
+        kind = ("verify", "verify", "verify", "sign", "poh")[t % 5]
+        if kind == "verify":
+            b = W.verify_plain(200 + 300 * t, oracle.sign, oracle.create_keypair, seed=40 + t,
+                               batch_sizes=[50, 150 + 300 * t])
+            want = _oracle_mask(oracle, b)
+        elif kind == "sign":
+            b = W.sign_batch(500 + 100 * t, oracle.create_keypair, seed=50 + t)
+            want = oracle.sign_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off)
+        else:
+            h, counts = W.poh_chains(300 + 50 * t, 200 + t, seed=60 + t)
+            want = h.copy()
+            oracle.poh_verify_many(want, counts, threads=4)
+            b = (h, counts)
+        jobs.append((kind, b, want))
+    results = [[] for _ in jobs]
+    errors = []
+
+    def run(i):
+        kind, b, _ = jobs[i]
+        try:
+            for _ in range(3):
+                if kind == "verify":
+                    got = gpu_lib.verify_many(b.batches, b.message_size, b.lens, b.pk_off,
+                                              b.sig_off, b.msg_off)
+                elif kind == "sign":
+                    got = gpu_lib.sign_many(b.batches, b.message_size, b.lens, b.pk_off,
+                                            b.sig_off, b.msg_off)
+                else:
+                    got = b[0].copy()
+                    assert gpu_lib.poh_verify_many(got, b[1]) == 0
+                results[i].append(got)
+        except Exception as exc:          # reported below, on the main thread
+            errors.append((i, repr(exc)))
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(len(jobs))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+        assert not th.is_alive(), "a caller did not return"
+    assert not errors, errors
+    for (kind, _, want), got in zip(jobs, results):
+        assert len(got) == 3
+        for g in got:
+            assert np.array_equal(g, want), kind
