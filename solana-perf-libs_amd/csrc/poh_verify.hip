@@ -10,7 +10,7 @@
 // Two kernels, by how many chains there are:
 //   poh_chain_kernel  one chain per lane.  Once the chains fill every SIMD
 //                     (64 per wave, one wave per SIMD) this is the fastest
-//                     form: a wave issues its 1,344 instructions per hash
+//                     form: a wave issues its 1,322 instructions per hash
 //                     back to back.
 //   poh_split_kernel  two waves per 64 chains, for fewer chains than that.
 //                     A wave64 VALU instruction occupies its SIMD for 4
@@ -19,8 +19,8 @@
 //                     the other SIMDs idle.  Here the second wave of the
 //                     workgroup (on another SIMD) computes the message
 //                     schedule W16..W63 (plus K) of the same hash and hands
-//                     it over through LDS, so the rounds wave issues ~970
-//                     instructions per hash instead of 1,344.  Four barriers
+//                     it over through LDS, so the rounds wave issues 881
+//                     instructions per hash instead of 1,322.  Four barriers
 //                     per hash keep the two waves in step; every one of them
 //                     sits in control flow that both waves run identically
 //                     (the hash count is the workgroup's maximum, and lanes
