@@ -139,51 +139,61 @@ class Watchdog:
 
 def in_flight_leg(lib, b, eng, mask, steps, depths=(2, 3)):
     """The same validator batch with several batches in flight: D engines,
-    each with its own buffers, stream and instantiated graph in the tuned
-    configuration, steps issued round-robin so up to D launches overlap (as
-    concurrent callers of the reference's ed25519_verify_many overlap on its
-    per-context streams).  One launch leaves a third of the SIMDs idle (651
-    waves on 1,024 SIMDs); overlapping launches fill them.  Reported beside
-    `value`, which times one launch per step."""
+    each with its own buffers, stream and instantiated graph, steps issued
+    round-robin so up to D launches overlap (as concurrent callers of the
+    reference's ed25519_verify_many overlap on its per-context streams).  One
+    launch leaves a third of the SIMDs idle (651 waves on 1,024 SIMDs);
+    overlapping launches fill them.  Timed in the tuned configuration and in
+    the one-shot half-scalar ones: a persistent grid claims every SIMD when it
+    launches, so one-shot launches overlap better.  Reported beside `value`,
+    which times one launch per step."""
     import hip_rt
     import perf_libs
-    extra = []
-    out = {"note": "whole-job sig-verifies/s over %d batches, D engines round-robin; "
-                   "every engine's mask checked against the tuned engine's" % steps}
+    engs = []
+    out = {"note": "whole-job sig-verifies/s over %d batches, D engines round-robin; every "
+                   "engine's mask checked against the tuned engine's" % steps, "configs": {}}
     try:
-        want = eng.config()
-        for _ in range(max(depths) - 1):
+        for _ in range(max(depths)):
             e = perf_libs.VerifyEngine(lib, b.n, b.packets.size)
+            engs.append(e)
             hip_rt.upload_to(e.d_packets, b.packets)
             for dptr, tab in ((e.d_lens, b.lens), (e.d_pk, b.pk_off), (e.d_sig, b.sig_off),
                               (e.d_msg, b.msg_off)):
                 hip_rt.upload_to(dptr, tab)
-            extra.append(e)
-            cands = e.candidates()
-            if want in cands:
-                e.use(cands.index(want))
-            else:
-                e.autotune(b.n)
-        for d in depths:
-            engs = [eng] + extra[:d - 1]
-            for e in engs:                       # warm-up: one launch each
-                assert e.run(b.n) == 0
+        cands = engs[0].candidates()
+        configs = [eng.config()] + ["one-shot W=4 scalars=half waves_per_eu=1",
+                                    "one-shot W=5 scalars=half waves_per_eu=1"]
+        best = None
+        for cfg in dict.fromkeys(c for c in configs if c in cands):
             for e in engs:
-                e.sync()
-            hip_rt.synchronize()
-            t0 = time.perf_counter()
-            for k in range(steps):
-                assert engs[k % d].run(b.n) == 0
-            for e in engs:
-                e.sync()
-            hip_rt.synchronize()
-            wall = time.perf_counter() - t0
-            exact = all(np.array_equal(hip_rt.download_from(e.d_out, b.n), mask) for e in engs)
-            out[f"depth_{d}"] = {"sig_verifies_per_s": steps * b.n / wall,
-                                 "ms_per_batch": wall * 1e3 / steps, "masks_equal": exact}
-            log(f"bench: {d} batches in flight: {steps * b.n / wall:.3e} sig/s, exact {exact}")
+                e.use(cands.index(cfg))
+            row = {}
+            for d in depths:
+                use = engs[:d]
+                for e in use:                        # warm-up: one launch each
+                    assert e.run(b.n) == 0
+                for e in use:
+                    e.sync()
+                hip_rt.synchronize()
+                t0 = time.perf_counter()
+                for k in range(steps):
+                    assert use[k % d].run(b.n) == 0
+                for e in use:
+                    e.sync()
+                hip_rt.synchronize()
+                wall = time.perf_counter() - t0
+                exact = all(np.array_equal(hip_rt.download_from(e.d_out, b.n), mask)
+                            for e in use)
+                rate = steps * b.n / wall
+                row[f"depth_{d}"] = {"sig_verifies_per_s": rate,
+                                     "ms_per_batch": wall * 1e3 / steps, "masks_equal": exact}
+                log(f"bench: {d} batches in flight ({cfg}): {rate:.3e} sig/s, exact {exact}")
+                if exact and (best is None or rate > best["sig_verifies_per_s"]):
+                    best = {"config": cfg, "depth": d, "sig_verifies_per_s": rate}
+            out["configs"][cfg] = row
+        out["best"] = best
     finally:
-        for e in extra:
+        for e in engs:
             e.close()
     return out
 
