@@ -88,6 +88,15 @@ __device__ __forceinline__ void sha256_schedule16(uint32_t (&w)[16], uint32_t (*
 
 constexpr uint32_t kPohSplitThreads = 128;   // wave 0: rounds, wave 1: schedule
 
+// Rounds in the rounds wave's first phase, the one that overlaps the schedule
+// wave reading the digest from LDS and computing W16..31.  Timed on 512 to
+// 32,768 chains: 14 is 0.7-1 % faster than 12 or 13 (the schedule wave's
+// first phase was the longer one) and than 15 or 16 (fewer rounds are left
+// to cover the LDS reads after each barrier).
+constexpr int kPohPhase1Rounds = 14;
+static_assert(kPohPhase1Rounds >= 9 && kPohPhase1Rounds <= 16,
+              "rounds 0..8 (state words and the padding bit) always run in phase 1");
+
 __global__ void __launch_bounds__(kPohSplitThreads)
 poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ counts, uint64_t n) {
     __shared__ uint32_t kw[48][64];              // K[t] + W[t], t = 16..63
@@ -117,20 +126,21 @@ poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ cou
     // rounds wave: K + W of t = 16..31, 32..47 and 48..63, one array per
     // phase so the four rounds carried into the next phase need no copies
     uint32_t qa[16], qb[16], qc[16];
-    // The rounds wave's phases end at rounds 12, 28, 44 and 64; right after
-    // each barrier it loads K + W for the rest of that phase and the first
-    // four rounds of the next one, so every LDS read has rounds of work in
-    // front of its first use.  The schedule wave writes W16..31, W32..47 and
-    // W48..63 in the first three phases and waits out the fourth.
+    // The rounds wave's phases end at rounds P1, P1 + 16, P1 + 32 and 64;
+    // right after each barrier it loads K + W for the rest of that phase and
+    // the next phase's first 16 - P1 rounds, so every LDS read has rounds of
+    // work in front of its first use.  The schedule wave writes W16..31,
+    // W32..47 and W48..63 in the first three phases and waits out the fourth.
+    constexpr int P1 = kPohPhase1Rounds;
     for (uint64_t r = 0; r < reps; r++) {
-        // phase 1: rounds 0..11 from the state | W16..31
+        // phase 1: rounds 0..P1-1 from the state | W16..31
         if (rounds) {
             v = Sha256Vars{H0[0], H0[1], H0[2], H0[3], H0[4], H0[5], H0[6], H0[7]};
 #pragma unroll
             for (int t = 0; t < 8; t++) sha256_round(v, st[t] + K[t]);
             sha256_round(v, 0x80000000u + K[8]);
 #pragma unroll
-            for (int t = 9; t < 12; t++) sha256_round(v, K[t]);
+            for (int t = 9; t < P1; t++) sha256_round(v, t == 15 ? 256u + K[15] : K[t]);
         } else {
 #pragma unroll
             for (int k = 0; k < 8; k++) w[k] = msg[k][lane];
@@ -141,40 +151,39 @@ poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ cou
             sha256_schedule16<16>(w, kw, lane);
         }
         __syncthreads();
-        // phase 2: rounds 12..27 | W32..47
+        // phase 2: rounds P1..P1+15 | W32..47
         if (rounds) {
 #pragma unroll
             for (int t = 16; t < 32; t++) qa[t - 16] = kw[t - 16][lane];
             __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the rounds
 #pragma unroll
-            for (int t = 12; t < 15; t++) sha256_round(v, K[t]);
-            sha256_round(v, 256u + K[15]);
+            for (int t = P1; t < 16; t++) sha256_round(v, t == 15 ? 256u + K[15] : K[t]);
 #pragma unroll
-            for (int t = 16; t < 28; t++) sha256_round(v, qa[t - 16]);
+            for (int t = 16; t < P1 + 16; t++) sha256_round(v, qa[t - 16]);
         } else {
             sha256_schedule16<32>(w, kw, lane);
         }
         __syncthreads();
-        // phase 3: rounds 28..43 | W48..63
+        // phase 3: rounds P1+16..P1+31 | W48..63
         if (rounds) {
 #pragma unroll
             for (int t = 32; t < 48; t++) qb[t - 32] = kw[t - 16][lane];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int t = 28; t < 32; t++) sha256_round(v, qa[t - 16]);
+            for (int t = P1 + 16; t < 32; t++) sha256_round(v, qa[t - 16]);
 #pragma unroll
-            for (int t = 32; t < 44; t++) sha256_round(v, qb[t - 32]);
+            for (int t = 32; t < P1 + 32; t++) sha256_round(v, qb[t - 32]);
         } else {
             sha256_schedule16<48>(w, kw, lane);
         }
         __syncthreads();
-        // phase 4: rounds 44..63, feed-forward, next message
+        // phase 4: rounds P1+32..63, feed-forward, next message
         if (rounds) {
 #pragma unroll
             for (int t = 48; t < 64; t++) qc[t - 48] = kw[t - 16][lane];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int t = 44; t < 48; t++) sha256_round(v, qb[t - 32]);
+            for (int t = P1 + 32; t < 48; t++) sha256_round(v, qb[t - 32]);
 #pragma unroll
             for (int t = 48; t < 64; t++) sha256_round(v, qc[t - 48]);
             if (r < cnt) {
