@@ -104,9 +104,18 @@ def run_cpu_baseline(b, seconds=10.0):
         raise RuntimeError("CPU baseline: passes over the same batch disagree")
     src = ("oracle/_ref: reference sources built with g++ -O3" if kind == "reference"
            else "oracle restatement")
+    # one thread, for the per-core rate: the first 4096 signatures of the batch
+    m = min(4096, b.n)
+    t = time.perf_counter()
+    one = impl.verify_many(b.packets, b.lens[:m], b.pk_off[:m], b.sig_off[:m], b.msg_off[:m])
+    dt1 = time.perf_counter() - t
+    if not np.array_equal(one, first[:m]):
+        raise RuntimeError("CPU baseline: the one-thread sample disagrees with the batch pass")
     return {"value": n / dt, "unit": "sig-verifies/s", "cores": threads, "kind": kind,
             "sample": f"{passes} passes over the validator-shaped batch ({n} signatures; "
-                      f"{src}), {threads} threads, {dt:.1f} s"}, first
+                      f"{src}), {threads} threads, {dt:.1f} s",
+            "one_thread": {"value": m / dt1, "sample": f"first {m} signatures of the batch, "
+                                                        f"1 thread, {dt1:.2f} s"}}, first
 
 
 class Watchdog:
