@@ -141,7 +141,8 @@ class Watchdog:
             if time.monotonic() - self.last > self.limit_s:
                 msg = f"no progress for {self.limit_s:.0f} s; leg abandoned"
                 log(f"bench: {self.key}: {msg}")
-                self.result[self.key] = {"error": msg}
+                # keep whatever the leg had already measured
+                self.result[self.key] = {**self.result.get(self.key, {}), "error": msg}
                 print(json.dumps(self.result), flush=True)
                 os._exit(0)
 
@@ -239,10 +240,12 @@ def abi_callers_leg(lib, b, mask, calls=8, thread_counts=(1, 2, 4)):
     return out
 
 
-def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None):
+def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None):
     """Served mixed load: Poisson verify + PoH requests through spl_server at
     stepped offered loads, beside back-to-back plain ed25519_verify_many calls
-    on the same requests.  progress() is called after each step."""
+    on the same requests.  progress() is called after each step; `partial`
+    (a dict) receives the steps measured so far as they complete."""
+    partial = {} if partial is None else partial
     import loadgen
     import perf_libs
     import workloads as W
@@ -269,6 +272,7 @@ def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None):
             row = {"offered_sigs_per_s": rate, **loadgen.summarize(recs, span),
                    "server": server.stats(reset=True)}
             steps.append(row)
+            partial["steps"] = steps
             progress()
             v = row["verify"]
             log(f"bench: served {rate:.1e} sig/s offered: {v['units_per_s']:.3e} sig/s, "
@@ -675,7 +679,9 @@ def main():
     # last, under a watchdog: every other figure is already in the result
     if not args.no_extra and not args.no_serve:
         dog = Watchdog(result, "served", limit_s=150.0)
-        result["served"] = served_leg(lib, b, mask, progress=dog.kick)
+        result["served"] = {"partial": True}
+        result["served"] = served_leg(lib, b, mask, progress=dog.kick,
+                                      partial=result["served"])
         dog.close()
     print(json.dumps(result), flush=True)
 
