@@ -369,31 +369,33 @@ int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
     std::string rep;
     int rc = 0;
     // Each candidate is timed as the engine runs it: its launch sequence
-    // captured and instantiated as the graph, and a step of spl_verify_engine_run
-    // (write n, replay the graph) between the events.  The first replay warms
-    // the caches; the best of the next three is kept.  Timing the kernels
-    // launched one by one instead undercounted the persistent forms, whose
-    // prepass kernels are separate graph nodes: one tied the one-shot form
-    // there and then ran 1.5 % slower per step.
+    // captured and instantiated as the graph, and steps of
+    // spl_verify_engine_run (write n, replay the graph) issued back to back:
+    // one to warm the caches, then four between the events, whose mean is the
+    // candidate's time.  Timing the kernels launched one at a time, or the
+    // best of single replays, favoured the persistent forms (their prepass
+    // kernels are separate graph nodes, and their steps vary more): one tied
+    // the one-shot form there and then ran back to back 1 % slower.
+    constexpr int kTimedSteps = 4;
     for (int i = 0; i < nc && rc == 0; i++) {
         const VerifyConfig &c = cand[i];
         eng->cfg = c;
         if (eng->capture() != hipSuccess) continue;
-        float ms = 1e30f;
-        for (int r = 0; r < 4 && rc == 0; r++) {
-            float t = 0;
-            if (hipEventRecord(e0, eng->stream) != hipSuccess) {
-                rc = 3;
-                break;
-            }
+        float ms = 0;
+        launch_set_n(eng->n_dev.as<uint32_t>(), n, eng->stream);
+        if (hipGraphLaunch(eng->exec, eng->stream) != hipSuccess ||
+            hipStreamSynchronize(eng->stream) != hipSuccess ||
+            hipEventRecord(e0, eng->stream) != hipSuccess)
+            rc = 3;
+        for (int r = 0; r < kTimedSteps && rc == 0; r++) {
             launch_set_n(eng->n_dev.as<uint32_t>(), n, eng->stream);
-            if (hipGraphLaunch(eng->exec, eng->stream) != hipSuccess ||
-                hipEventRecord(e1, eng->stream) != hipSuccess ||
-                hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&t, e0, e1) != hipSuccess)
-                rc = 3;
-            if (r > 0 && t < ms) ms = t;
+            if (hipGraphLaunch(eng->exec, eng->stream) != hipSuccess) rc = 3;
         }
+        if (rc == 0 && (hipEventRecord(e1, eng->stream) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = 3;
+        ms /= kTimedSteps;
         if (rc) break;
         char name[128], line[160];
         config_name(c, name, sizeof name);
