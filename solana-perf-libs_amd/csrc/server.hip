@@ -95,6 +95,94 @@ struct LatencyStats {
     void reset() { *this = LatencyStats(); }
 };
 
+// Hardware resources of servers, kept for the life of the process and handed
+// to the next server created on the same device: a slot's stream (CU-masked,
+// so a hardware queue of its own), its completion event and its verify
+// scratch, and a ring's device buffer and pinned mirror.  Creating a server
+// then makes these only when no free ones exist, and destroying it makes no
+// HIP call at all: it only hands them back.
+struct SlotRes {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    bool in_use = false;
+};
+
+struct RingRes {
+    int dev = 0;
+    uint64_t cap = 0;
+    uint8_t *d = nullptr, *h = nullptr;
+    bool in_use = false;
+};
+
+std::mutex g_res_mu;
+std::vector<SlotRes *> g_slot_res;
+std::vector<RingRes *> g_ring_res;
+
+SlotRes *take_slot_res(int dev, const std::vector<uint32_t> &cu_mask) {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    for (SlotRes *r : g_slot_res)
+        if (r->dev == dev && !r->in_use) {
+            r->in_use = true;
+            return r;
+        }
+    auto *r = new SlotRes();
+    r->dev = dev;
+    // A stream created with a CU mask gets a hardware queue of its own rather
+    // than one shared round-robin with the process's other streams, so a long
+    // PoH kernel never sits in front of a verify batch.
+    if (hipExtStreamCreateWithCUMask(&r->stream, (uint32_t)cu_mask.size(), cu_mask.data()) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess) {
+        if (r->stream) (void)hipStreamDestroy(r->stream);
+        delete r;
+        return nullptr;
+    }
+    r->in_use = true;
+    g_slot_res.push_back(r);
+    return r;
+}
+
+// A free ring of at least `cap` bytes on `dev`, else a new one.
+RingRes *take_ring_res(int dev, uint64_t cap) {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    for (RingRes *r : g_ring_res)
+        if (r->dev == dev && !r->in_use && r->cap >= cap) {
+            r->in_use = true;
+            return r;
+        }
+    auto *r = new RingRes();
+    r->dev = dev;
+    r->cap = cap;
+    // coherent: the egress kernel's writes are visible to the host once the
+    // batch's event has fired
+    if (hipMalloc(&r->d, cap) != hipSuccess ||
+        hipHostMalloc(&r->h, cap, hipHostMallocCoherent) != hipSuccess) {
+        if (r->d) (void)hipFree(r->d);
+        delete r;
+        return nullptr;
+    }
+    // Touch every page of the mirror now: the first CPU write to a page of
+    // pinned memory was measured at 2.7 GB/s, later ones at 33-52 GB/s, and
+    // staging must not pay that on the request path.
+    memset(r->h, 0, cap);
+    r->in_use = true;
+    g_ring_res.push_back(r);
+    return r;
+}
+
+void give_back(SlotRes *r) {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    r->in_use = false;
+}
+
+void give_back(RingRes *r) {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    r->in_use = false;
+}
+
 }  // namespace
 }  // namespace spl
 
@@ -103,16 +191,14 @@ using namespace spl;
 struct spl_server {
     struct Slot {
         Kind kind = kVerify;
-        hipStream_t stream = nullptr;
-        hipEvent_t done = nullptr;
-        void *scratch = nullptr;
-        size_t scratch_bytes = 0;
+        SlotRes *r = nullptr;                 // stream, event, scratch (pooled)
         Batch *batch = nullptr;
     };
 
     int dev = 0, cus = 256;
     uint32_t max_batch = 0;
     Ring ring;
+    RingRes *ring_res = nullptr;              // the ring's memory (pooled)
     uint8_t *d_ring = nullptr, *h_ring = nullptr;
     const ed25519_tables *T = nullptr;
     std::vector<Slot> slots;
@@ -130,14 +216,12 @@ struct spl_server {
     LatencyStats stats[2];
     uint32_t inflight = 0, max_inflight = 0;
 
+    // Every batch has completed by now (the loop only exits idle), so the
+    // pooled resources go back with no work on them.
     ~spl_server() {
-        for (auto &s : slots) {
-            if (s.done) (void)hipEventDestroy(s.done);
-            if (s.stream) (void)hipStreamDestroy(s.stream);
-            if (s.scratch) (void)hipFree(s.scratch);
-        }
-        if (d_ring) (void)hipFree(d_ring);
-        if (h_ring) (void)hipHostFree(h_ring);
+        for (auto &s : slots)
+            if (s.r) give_back(s.r);
+        if (ring_res) give_back(ring_res);
         for (auto &kv : reqs) delete kv.second;
     }
 
@@ -241,16 +325,17 @@ struct spl_server {
     }
 
     hipError_t ensure_scratch(Slot &s, size_t bytes) {
-        if (bytes <= s.scratch_bytes) return hipSuccess;
-        if (s.scratch) {
-            hipError_t e = hipStreamSynchronize(s.stream);
+        SlotRes &r = *s.r;
+        if (bytes <= r.scratch_bytes) return hipSuccess;
+        if (r.scratch) {
+            hipError_t e = hipStreamSynchronize(r.stream);
             if (e != hipSuccess) return e;
-            (void)hipFree(s.scratch);
-            s.scratch = nullptr;
-            s.scratch_bytes = 0;
+            (void)hipFree(r.scratch);
+            r.scratch = nullptr;
+            r.scratch_bytes = 0;
         }
-        hipError_t e = hipMalloc(&s.scratch, bytes);
-        if (e == hipSuccess) s.scratch_bytes = bytes;
+        hipError_t e = hipMalloc(&r.scratch, bytes);
+        if (e == hipSuccess) r.scratch_bytes = bytes;
         return e;
     }
 
@@ -266,25 +351,25 @@ struct spl_server {
             const VerifyConfig cfg = tuned_plain_config(dev, b.n);
             if ((e = ensure_scratch(s, verify_scratch_bytes(cfg, b.n, cus))) != hipSuccess)
                 return e;
-            if ((e = launch_copy16(d, h, b.in_bytes, s.stream)) != hipSuccess) return e;
+            if ((e = launch_copy16(d, h, b.in_bytes, s.r->stream)) != hipSuccess) return e;
             const uint64_t tab_off = align_up(b.packets_bytes, 256);
             const uint32_t *tab = reinterpret_cast<const uint32_t *>(d + tab_off);
             VerifyJob job = make_verify_job(d, b.packets_bytes, tab, tab + b.n, tab + 2ull * b.n,
-                                            tab + 3ull * b.n, d + b.out_off, s.scratch, T);
-            if ((e = launch_verify_cfg(cfg, job, b.n, cus, s.stream)) != hipSuccess) return e;
+                                            tab + 3ull * b.n, d + b.out_off, s.r->scratch, T);
+            if ((e = launch_verify_cfg(cfg, job, b.n, cus, s.r->stream)) != hipSuccess) return e;
         } else {
             stage_poh(b);
-            if ((e = launch_copy16(d, h, b.in_bytes, s.stream)) != hipSuccess) return e;
+            if ((e = launch_copy16(d, h, b.in_bytes, s.r->stream)) != hipSuccess) return e;
             if ((e = launch_poh(reinterpret_cast<uint32_t *>(d),
                                 reinterpret_cast<const uint64_t *>(d + 32ull * b.n), b.n,
-                                s.stream)) != hipSuccess)
+                                s.r->stream)) != hipSuccess)
                 return e;
         }
-        if ((e = launch_copy16(h + b.out_off, d + b.out_off, b.out_bytes, s.stream)) !=
+        if ((e = launch_copy16(h + b.out_off, d + b.out_off, b.out_bytes, s.r->stream)) !=
             hipSuccess)
             return e;
         b.t_launched = now_ns();
-        return hipEventRecord(s.done, s.stream);
+        return hipEventRecord(s.r->done, s.r->stream);
     }
 
     // Delivers a finished batch's results.  Called without mu.
@@ -335,7 +420,7 @@ struct spl_server {
             bool progress = false;
             for (Slot &s : slots) {
                 if (!s.batch) continue;
-                const hipError_t q = hipEventQuery(s.done);
+                const hipError_t q = hipEventQuery(s.r->done);
                 if (q == hipErrorNotReady) continue;
                 Batch *b = s.batch;
                 b->t_device_done = now_ns();
@@ -477,25 +562,19 @@ spl_server *spl_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch_sign
         sv->cus = device_cus(sv->dev);
         sv->max_batch = max_batch_signatures;
         sv->ring = Ring(align_up(hbm_budget_bytes, 256));
-        ok = hipMalloc(&sv->d_ring, sv->ring.cap()) == hipSuccess &&
-             // coherent: the egress kernel's writes are visible to the host once
-             // the batch's event has fired
-             hipHostMalloc(&sv->h_ring, sv->ring.cap(), hipHostMallocCoherent) == hipSuccess;
-        // Touch every page of the mirror now: the first CPU write to a page of
-        // pinned memory was measured at 2.7 GB/s, later ones at 33-52 GB/s,
-        // and staging must not pay that on the request path.
-        if (ok) memset(sv->h_ring, 0, sv->ring.cap());
+        sv->ring_res = take_ring_res(sv->dev, sv->ring.cap());
+        ok = sv->ring_res != nullptr;
+        if (ok) {
+            sv->d_ring = sv->ring_res->d;
+            sv->h_ring = sv->ring_res->h;
+        }
     }
     std::vector<uint32_t> all_cus((sv->cus + 31) / 32, 0xFFFFFFFFu);
     for (uint32_t k = 0; ok && k < verify_streams + poh_streams; k++) {
         spl_server::Slot s;
         s.kind = k < verify_streams ? kVerify : kPoh;
-        // A stream created with a CU mask gets a hardware queue of its own
-        // rather than one shared round-robin with the process's other
-        // streams, so a long PoH kernel never sits in front of a verify batch.
-        ok = hipExtStreamCreateWithCUMask(&s.stream, (uint32_t)all_cus.size(), all_cus.data()) ==
-                 hipSuccess &&
-             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) == hipSuccess;
+        s.r = take_slot_res(sv->dev, all_cus);
+        ok = s.r != nullptr;
         if (ok && s.kind == kVerify) {
             // sized for a full batch under the widest window, so the loop
             // never allocates in steady state
@@ -503,7 +582,7 @@ spl_server *spl_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch_sign
             ok = sv->ensure_scratch(s, verify_scratch_bytes(widest, max_batch_signatures,
                                                             sv->cus)) == hipSuccess;
         }
-        sv->slots.push_back(s);
+        if (s.r) sv->slots.push_back(s);
     }
     if (!ok) {
         fprintf(stderr, "spl_server_create: failed\n");
