@@ -10,11 +10,18 @@ second over K timed steps.  Also reported (same JSON line):
                  saturating 262,144 x 500 run
   plain_verify:  the plain ed25519_verify_many ABI (host buffers, PCIe in and
                  out) on 1024 one-signature packets: p50/p99 call latency
-  roofline:      verify kernel, int32 VALU ops per second vs 3.93e13
+  roofline:      verify kernel, issued int32 VALU lane-ops per second vs
+                 3.93e13 (frac), and the reference algorithm's op count over
+                 the same time (ref_equiv_frac)
   cpu_baseline:  the reference's own CPU code (oracle/_ref, built from its
                  sources) on the same batch, all host cores
 Inputs are synthetic and seeded; signing for input preparation uses the
 library under test.
+
+Output: one JSON line (under 4 KB) as the last line of stdout; every table
+behind it (autotune candidates, served steps, batches in flight, callers)
+goes to gpurun_out/bench_detail.json.  A served leg that stalls ends the run
+with exit status 3 after printing the line measured so far.
 """
 import argparse
 import json
@@ -58,8 +65,33 @@ def measured_row(launch_config):
     return None
 
 
+DETAIL_PATH = os.path.join(ROOT, "gpurun_out", "bench_detail.json")
+_NOTES = []
+
+
 def log(*a):
+    """A progress line on stderr (one per leg, so the whole run stays short)."""
     print(*a, file=sys.stderr, flush=True)
+    _NOTES.append(" ".join(str(x) for x in a))
+
+
+def note(*a):
+    """A detail line: kept for the side file only."""
+    _NOTES.append(" ".join(str(x) for x in a))
+
+
+def poh_issued_per_hash(kernel):
+    """Issued VALU lane-ops per chain-hash of a PoH kernel (SQ_INSTS_VALU x 64
+    per hash step of 64 chains, both waves of the split kernel summed), as
+    recorded in profiles/verify_poh_sign_bench.md; None when absent."""
+    path = os.path.join(ROOT, "profiles", "verify_poh_sign_bench.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == kernel and "per 64-chain hash" in cells[4]:
+            return float(cells[4].split()[0].replace(",", ""))
+    return None
 
 
 def cpu_threads():
@@ -119,13 +151,13 @@ def run_cpu_baseline(b, seconds=10.0):
 
 
 class Watchdog:
-    """Ends the run with the result line gathered so far when the leg it
-    watches makes no progress for limit_s seconds.  The serving loop went
-    silent twice on the GPU with no cause found; a stall there must not cost
-    the run its measured result."""
+    """Ends the run when the leg it watches makes no progress for limit_s
+    seconds: the result line gathered so far is printed (the leg marked with
+    its error) and the process exits with status 3, so a stall is recorded as
+    a failed run and never as a clean one."""
 
-    def __init__(self, result, key, limit_s):
-        self.result, self.key, self.limit_s = result, key, limit_s
+    def __init__(self, emit, result, key, limit_s):
+        self.emit, self.result, self.key, self.limit_s = emit, result, key, limit_s
         self.last = time.monotonic()
         self.done = threading.Event()
         threading.Thread(target=self._watch, daemon=True).start()
@@ -143,8 +175,8 @@ class Watchdog:
                 log(f"bench: {self.key}: {msg}")
                 # keep whatever the leg had already measured
                 self.result[self.key] = {**self.result.get(self.key, {}), "error": msg}
-                print(json.dumps(self.result), flush=True)
-                os._exit(0)
+                self.emit(self.result)
+                os._exit(3)
 
 
 def in_flight_leg(lib, b, eng, mask, steps, depths=(2, 3)):
@@ -197,7 +229,7 @@ def in_flight_leg(lib, b, eng, mask, steps, depths=(2, 3)):
                 rate = steps * b.n / wall
                 row[f"depth_{d}"] = {"sig_verifies_per_s": rate,
                                      "ms_per_batch": wall * 1e3 / steps, "masks_equal": exact}
-                log(f"bench: {d} batches in flight ({cfg}): {rate:.3e} sig/s, exact {exact}")
+                note(f"bench: {d} batches in flight ({cfg}): {rate:.3e} sig/s, exact {exact}")
                 if exact and (best is None or rate > best["sig_verifies_per_s"]):
                     best = {"config": cfg, "depth": d, "sig_verifies_per_s": rate}
             out["configs"][cfg] = row
@@ -235,8 +267,8 @@ def abi_callers_leg(lib, b, mask, calls=8, thread_counts=(1, 2, 4)):
         wall = time.perf_counter() - t0
         out[f"threads_{t}"] = {"sig_verifies_per_s": t * calls * b.n / wall,
                                "ms_per_call": wall * 1e3 / calls, "masks_exact": all(ok)}
-        log(f"bench: ed25519_verify_many from {t} threads: {t * calls * b.n / wall:.3e} sig/s, "
-            f"exact {all(ok)}")
+        note(f"bench: ed25519_verify_many from {t} threads: {t * calls * b.n / wall:.3e} sig/s, "
+             f"exact {all(ok)}")
     return out
 
 
@@ -275,7 +307,7 @@ def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None):
             partial["steps"] = steps
             progress()
             v = row["verify"]
-            log(f"bench: served {rate:.1e} sig/s offered: {v['units_per_s']:.3e} sig/s, "
+            note(f"bench: served {rate:.1e} sig/s offered: {v['units_per_s']:.3e} sig/s, "
                 f"p50 {v['p50_ms']:.2f} ms, p99 {v['p99_ms']:.2f} ms, "
                 f"poh p99 {row.get('poh', {}).get('p99_ms', 0):.1f} ms, exact "
                 f"{v['all_exact'] and row.get('poh', {'all_exact': True})['all_exact']}")
@@ -363,7 +395,7 @@ def packets_leg(lib, stream, reps=10):
     host_exact = rc == 0 and bool(np.array_equal(ok, want_ok))
     first_ms = lat[0]
     lat = np.array(lat[9:])
-    log(f"bench: packets {num}: parse {parse_ms * 1e3:.1f} us, pipeline {pipe_ms:.3f} ms")
+    note(f"bench: packets {num}: parse {parse_ms * 1e3:.1f} us, pipeline {pipe_ms:.3f} ms")
     return {"workload": "wire-format packets, validator-shaped (128 batches x U[1,256] packets, "
                         "1-4 sigs, 50% version-0, 2% malformed), stride 1232 B",
             "packets": num, "signatures": n, "accepted_packets": int(want_ok.sum()),
@@ -506,7 +538,8 @@ def main():
                       (eng.d_msg, b.msg_off)):
         hip_rt.upload_to(dptr, tab)
     tune_report = eng.autotune(b.n) if not args.no_tune else "(not tuned)"
-    log("bench: autotune\n" + tune_report + "bench: chosen " + eng.config())
+    note("bench: autotune\n" + tune_report)
+    log("bench: autotuned; chosen " + eng.config())
 
     def step():
         assert eng.run(b.n) == 0
@@ -530,7 +563,9 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = b.n / (ms_per_step / 1e3)
     ops = verify_ops(b.lens)
-    achieved = ops / (kern_ms / 1e3)
+    row = measured_row(eng.config()) or {}
+    issued = row.get("valu_lane_ops_per_sig")
+    launch_s = kern_ms / 1e3
     result = {
         "metric": "sig verifies/s + PoH SHA-256 hashes/s on 1 MI355X; p99 request latency",
         "value": value, "unit": "sig-verifies/s", "n_gpus": 1, "steps": args.steps,
@@ -545,15 +580,25 @@ def main():
                    "signatures": b.n, "packets": b.total_packets,
                    "accepted": int(mask.sum()), "launch_config": eng.config(),
                    "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
-                     "unit": "TOP/s (int32 lane-ops)", "frac": achieved / VALU_PEAK,
-                     "traffic": (measured_row(eng.config()) or {}).get("traffic"),
-                     "issued_valu_lane_ops_per_sig":
-                         (measured_row(eng.config()) or {}).get("valu_lane_ops_per_sig"),
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch, "
-                                       "profiles/verify_poh_sign_bench.md",
-                     "kernel": "verify kernel (tuned configuration)",
-                     "ops_per_launch": ops, "launch_ms": kern_ms},
+        # frac: the kernel's issued VALU work (SQ_INSTS_VALU x 64 lane-ops per
+        # launch of this configuration, from the committed counter run) over
+        # the launch time measured here, against the int32 VALU peak.
+        # ref_equiv_frac: the reference algorithm's int32 op count for the same
+        # signatures (full 253-bit scalars, radix-2^32 field arithmetic) over
+        # the same time; it exceeds 1 because this kernel does the same
+        # verification with half as many issued operations.
+        "roofline": {"bound": "valu", "unit": "TOP/s (int32 lane-ops)",
+                     "achieved": None if issued is None else issued * b.n / launch_s / 1e12,
+                     "peak": VALU_PEAK / 1e12,
+                     "frac": None if issued is None else issued * b.n / launch_s / VALU_PEAK,
+                     "ref_equiv_frac": ops / launch_s / VALU_PEAK,
+                     "traffic": row.get("traffic"),
+                     "algorithmic_bytes": int(np.sum(113 + b.lens.astype(np.int64)) + b.n),
+                     "launch_ms": kern_ms,
+                     "issued_lane_ops_per_sig": issued,
+                     "source": "issued ops and traffic (FETCH_SIZE x2.0 + WRITE_SIZE x1.0) per "
+                               "launch of this configuration: rocprofv3 counters in "
+                               "profiles/verify_poh_sign_bench.md; launch_ms: HIP events here"},
         "autotune": [l for l in tune_report.splitlines() if l.strip()],
     }
 
@@ -598,8 +643,11 @@ def main():
             e1.synchronize()
             ms = e0.elapsed_ms(e1)
             hps = n * reps / (ms / 1e3)
-            poh[name] = {"hashes_per_s": hps, "ms": ms,
-                         "roofline_frac": hps * W_SHA256 / VALU_PEAK}
+            kern = "poh_split_kernel" if n <= 32768 else "poh_chain_kernel"
+            per = poh_issued_per_hash(kern)
+            poh[name] = {"hashes_per_s": hps, "ms": ms, "kernel": kern,
+                         "issued_frac": None if per is None else hps * per / VALU_PEAK,
+                         "ref_equiv_frac": hps * W_SHA256 / VALU_PEAK}
             log(f"bench: poh {name}: {hps:.3e} hashes/s")
         result["poh"] = poh
         # plain ABI call latency (host buffers, PCIe both ways)
@@ -678,12 +726,87 @@ def main():
 
     # last, under a watchdog: every other figure is already in the result
     if not args.no_extra and not args.no_serve:
-        dog = Watchdog(result, "served", limit_s=150.0)
+        dog = Watchdog(emit, result, "served", limit_s=150.0)
         result["served"] = {"partial": True}
         result["served"] = served_leg(lib, b, mask, progress=dog.kick,
                                       partial=result["served"])
         dog.close()
-    print(json.dumps(result), flush=True)
+    emit(result)
+
+
+def _get(d, *path):
+    for k in path:
+        if not isinstance(d, dict) or k not in d:
+            return None
+        d = d[k]
+    return d
+
+
+def compact(r):
+    """The one result line: the headline, its roofline and CPU baseline, and
+    one figure per other leg.  Every table behind them is in the side file."""
+    line = {k: r[k] for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup",
+                              "ms_per_step", "higher_is_better", "scaling", "vs_baseline",
+                              "dtype", "data", "config", "roofline") if k in r}
+    line["roofline"] = {k: v for k, v in line.get("roofline", {}).items()
+                        if k not in ("source",)}
+    if "cpu_baseline" in r:
+        cb = dict(r["cpu_baseline"])
+        one = cb.pop("one_thread", None)
+        if one:
+            cb["one_thread_value"] = one["value"]
+        line["cpu_baseline"] = cb
+        line["parity_vs_cpu"] = r.get("parity_vs_cpu")
+    poh = r.get("poh")
+    if poh:
+        line["poh"] = {name: {"hashes_per_s": v["hashes_per_s"], "ms": v["ms"],
+                              "issued_frac": v["issued_frac"],
+                              "ref_equiv_frac": v["ref_equiv_frac"]} for name, v in poh.items()}
+    best = _get(r, "in_flight", "best")
+    if best:
+        line["in_flight_best"] = best
+    elif _get(r, "in_flight", "error"):
+        line["in_flight_best"] = {"error": r["in_flight"]["error"][:200]}
+    sv = r.get("served")
+    if sv:
+        steps = sv.get("steps") or []
+        line["served"] = {
+            "best_at_p99_le_2ms": sv.get("best_served_at_p99_le_2ms"),
+            "back_to_back_plain_sig_per_s": _get(sv, "back_to_back_plain", "sig_verifies_per_s"),
+            "steps": len(steps),
+            "all_exact": all(s["verify"]["all_exact"] and s.get("poh", {"all_exact": True})
+                             ["all_exact"] for s in steps),
+        }
+        if sv.get("error"):
+            line["served"]["error"] = str(sv["error"])[:300]
+    for key, path in (("sign_per_s", ("sign_65536", "signs_per_s")),
+                      ("plain_verify_1024_p99_ms", ("plain_verify_1024", "p99_ms")),
+                      ("packets_pipeline_per_s", ("packets", "pipeline_packets_per_s"))):
+        v = _get(r, *path)
+        if v is not None:
+            line[key] = v
+    line["detail"] = os.path.relpath(DETAIL_PATH, ROOT)
+    return line
+
+
+def emit(result):
+    """Writes every table to the side file and prints the one result line
+    (kept under 4 KB) as the last line of stdout."""
+    try:
+        os.makedirs(os.path.dirname(DETAIL_PATH), exist_ok=True)
+        with open(DETAIL_PATH, "w") as f:
+            json.dump({**result, "log": _NOTES}, f, indent=1)
+    except OSError as exc:
+        log(f"bench: side file not written: {exc!r}")
+    line = json.dumps(compact(result))
+    if len(line) > 4000:                        # never let the headline be cut off
+        c = compact(result)
+        for k in ("served", "poh", "in_flight_best", "cpu_baseline"):
+            c.pop(k, None)
+            line = json.dumps(c)
+            if len(line) <= 4000:
+                break
+    print(line, flush=True)
 
 
 if __name__ == "__main__":

@@ -12,6 +12,9 @@ Sources:
                        prior output buffers, the raw-limb small-order test on
                        the 8-torsion and non-canonical encodings plus random
                        encodings, and verify/sign/keypair on seeded cases.
+  ref_torsion.json     verify verdicts of the same library on keys A + T,
+                       R + T and both (T of order dividing 8), non-canonical
+                       keys and R (y >= p), and S in [2^252, L).
 Requires /root/reference and oracle/_ref/libref.so; the fixtures are committed
 so tests never need either.
 """
@@ -130,6 +133,115 @@ def ref_quirks():
             "get_checked_scalar": gcs, "small_order": so, "cases": cases}
 
 
+def ref_torsion():
+    """Verdicts of the compiled reference (verify.cu:109-174 over ge.cu) on the
+    classes where an exact check and a cofactored one part ways, plus the
+    encodings the reference's decoder accepts leniently:
+
+      a_plus_t     key A + T, signed over the encoding of A + T (h hashes it):
+                   [S]B - [h](A + T) = R - [h]T, so the reference accepts
+                   exactly when [h]T = O.  For each of the 8 points T of order
+                   dividing 8, rows with [h]T = O and rows without.
+      r_plus_t     R + T published in place of R: accepted only for T = O.
+      mixed        key A + T and R + T': accepted exactly when T' = -[h]T.
+      noncanon_a   key = y + p (y < 19, y + p < 2^255, either sign bit), a
+                   non-canonical encoding the decoder accepts; signature by
+                   another key.
+      noncanon_r   the same encodings as R.
+      s_high       S in [2^252, L): a canonical scalar the reference's nibble
+                   gate rejects.
+    Signatures are made with Python integers (tests/ed25519_math.py); the
+    verdict is the compiled reference's, and the closed form each class
+    predicts is checked against it here before anything is written."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ed25519_math as E
+    from oracle import RefLib
+    import workloads as W
+    ref = RefLib()
+    rng = np.random.default_rng(20241204)
+    tors = [E.decode_lenient(t) for t in W.TORSION_CANONICAL]
+    orders = [E.order(t) for t in tors]
+    rows = []
+
+    def key():
+        seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        a, prefix = E.expand(seed)
+        return a, prefix, E.mul(a, E.B)
+
+    def message():
+        return rng.integers(0, 256, int(rng.integers(0, 200)), dtype=np.uint8).tobytes()
+
+    def sign(a, prefix, pk, msg, r_shift=None):
+        r = E.sha512_int(prefix, msg) % E.L
+        Rp = E.mul(r, E.B)
+        if r_shift is not None:
+            Rp = E.add(Rp, r_shift)
+        R = E.encode(Rp)
+        h = E.sha512_int(R, pk, msg) % E.L
+        return R + ((r + h * a) % E.L).to_bytes(32, "little"), h
+
+    def add_row(cls, sig, msg, pk, want, **extra):
+        got = ref.verify(sig, msg, pk)
+        assert got == want, (cls, extra, got, want)
+        rows.append({"class": cls, "sig": sig.hex(), "msg": msg.hex(), "pk": pk.hex(),
+                     "verify": got, **extra})
+
+    for k, (T, o) in enumerate(zip(tors, orders)):
+        quota = {True: 6, False: 6} if o > 1 else {True: 12, False: 0}
+        while quota[True] or quota[False]:
+            a, prefix, A = key()
+            pk = E.encode(E.add(A, T))
+            msg = message()
+            sig, h = sign(a, prefix, pk, msg)
+            acc = h % o == 0
+            if quota[acc]:
+                quota[acc] -= 1
+                add_row("a_plus_t", sig, msg, pk, int(acc), t=k, t_order=o)
+        for _ in range(8):
+            a, prefix, A = key()
+            pk = E.encode(A)
+            msg = message()
+            sig, _ = sign(a, prefix, pk, msg, r_shift=T)
+            add_row("r_plus_t", sig, msg, pk, int(o == 1), t=k, t_order=o)
+    quota = {True: 24, False: 24}
+    while quota[True] or quota[False]:
+        k1, k2 = int(rng.integers(1, 8)), int(rng.integers(1, 8))
+        a, prefix, A = key()
+        pk = E.encode(E.add(A, tors[k1]))
+        msg = message()
+        sig, h = sign(a, prefix, pk, msg, r_shift=tors[k2])
+        acc = E.eq(E.add(tors[k2], E.mul(h % 8, tors[k1])), E.IDENT)
+        if quota[acc]:
+            quota[acc] -= 1
+            add_row("mixed", sig, msg, pk, int(acc), t=k1, t_r=k2)
+    noncanon = []
+    for y in range(2, 19):
+        for sign_bit in (0, 1):
+            enc = (E.P + y + (sign_bit << 255)).to_bytes(32, "little")
+            if E.decode_lenient(enc) is not None:
+                noncanon.append(enc)
+    for enc in noncanon:
+        a, prefix, A = key()
+        msg = message()
+        sig, _ = sign(a, prefix, E.encode(A), msg)
+        add_row("noncanon_a", sig, msg, enc, 0)
+        add_row("noncanon_r", enc + sig[32:], msg, E.encode(A), 0)
+    for j in range(20):
+        a, prefix, A = key()
+        pk = E.encode(A)
+        msg = message()
+        sig, _ = sign(a, prefix, pk, msg)
+        s = int(rng.integers(0, 2**62)) * (E.L - 2**252) // 2**62 + 2**252 if j % 2 else \
+            int.from_bytes(sig[32:], "little") + 2**252
+        assert 2**252 <= s < 2**256
+        add_row("s_high", sig[:32] + s.to_bytes(32, "little"), msg, pk, 0)
+    return {"source": "oracle/_ref/libref.so built from src/cuda-ecc-ed25519 "
+                      "(fe.cu ge.cu sc.cu sha512.cu keypair.cu); verify composition "
+                      "verify.cu:109-174",
+            "torsion_encodings": [t.hex() for t in W.TORSION_CANONICAL],
+            "rows": rows}
+
+
 def main():
     with open(os.path.join(OUT, "poh_reference.json"), "w") as f:
         json.dump(poh_fixture(), f, indent=0)
@@ -137,6 +249,8 @@ def main():
         json.dump(RFC8032, f, indent=1)
     with open(os.path.join(OUT, "ref_quirks.json"), "w") as f:
         json.dump(ref_quirks(), f, indent=0)
+    with open(os.path.join(OUT, "ref_torsion.json"), "w") as f:
+        json.dump(ref_torsion(), f, indent=0)
     print("fixtures written to", OUT)
 
 

@@ -160,6 +160,69 @@ def test_every_configuration(gpu_lib, oracle):
         eng.close()
 
 
+def _fixture_batch(rows, stride=512):
+    """The rows of a verify fixture as one batch, one signature per packet:
+    [0,64) signature, [64,96) key, [96,96+len) message."""
+    n = len(rows)
+    buf = np.zeros((n, stride), np.uint8)
+    lens = np.zeros(n, np.uint32)
+    for i, r in enumerate(rows):
+        sig, msg, pk = (np.frombuffer(bytes.fromhex(r[k]), np.uint8) for k in ("sig", "msg", "pk"))
+        assert 96 + msg.size <= stride
+        buf[i, :64], buf[i, 64:96], buf[i, 96:96 + msg.size] = sig, pk, msg
+        lens[i] = msg.size
+    base = np.arange(n, dtype=np.uint32) * stride
+    flat = buf.reshape(-1)
+    return W.Batch(flat, stride, [flat], lens, base + 64, base.copy(), base + 96)
+
+
+def test_every_configuration_torsion_fixture(gpu_lib):
+    """Every engine configuration, the drop-in ed25519_verify_many and the
+    device entry point give the compiled reference's own verdicts (not the
+    oracle's) on keys A + T, R + T and both, non-canonical keys and R, and
+    S in [2^252, L): the rows where a cofactored or torsion-blind check would
+    differ from the reference's exact one."""
+    import hip_rt
+    import perf_libs
+    from conftest import load_json
+    rows = load_json("ref_torsion.json")["rows"]
+    want = np.array([r["verify"] for r in rows], np.uint8)
+    assert 0 < want.sum() < want.size
+    b = _fixture_batch(rows)
+    # tiled, so each configuration also sees the rows at every lane position of a wave
+    reps = 5
+    packets = np.tile(b.packets, reps)
+    shift = np.repeat(np.arange(reps, dtype=np.uint32) * np.uint32(b.packets.size), b.n)
+    big = W.Batch(packets, b.message_size, [packets], np.tile(b.lens, reps),
+                  np.tile(b.pk_off, reps) + shift, np.tile(b.sig_off, reps) + shift,
+                  np.tile(b.msg_off, reps) + shift)
+    big_want = np.tile(want, reps)
+    eng = perf_libs.VerifyEngine(gpu_lib, big.n, big.packets.size)
+    try:
+        for k, name in enumerate(eng.candidates()):
+            eng.use(k)
+            for bb, ww in ((b, want), (big, big_want)):
+                _load(eng, bb)
+                assert eng.run(bb.n) == 0 and eng.sync() == 0
+                got = _mask(eng, bb.n)
+                bad = np.nonzero(got != ww)[0]
+                assert bad.size == 0, (name, [(rows[i % b.n]["class"], int(ww[i])) for i in bad[:8]])
+    finally:
+        eng.close()
+    got = gpu_lib.verify_many(big.batches, big.message_size, big.lens, big.pk_off, big.sig_off,
+                              big.msg_off)
+    assert np.array_equal(got, big_want)
+    d_p = hip_rt.DeviceBuffer.from_array(big.packets)
+    d_t = [hip_rt.DeviceBuffer.from_array(x) for x in (big.lens, big.pk_off, big.sig_off,
+                                                       big.msg_off)]
+    d_o = hip_rt.DeviceBuffer(big.n)
+    s = hip_rt.Stream()
+    assert gpu_lib.verify_many_device(d_p.ptr, big.packets.size, *[x.ptr for x in d_t], big.n,
+                                      d_o.ptr, s.handle) == 0
+    s.synchronize()
+    assert np.array_equal(d_o.download(), big_want)
+
+
 def test_every_configuration_full_batches(gpu_lib, oracle):
     """Every candidate configuration against the oracle on full-size
     validator-shaped batches (about 41k signatures each: 5 % flipped bits,

@@ -140,6 +140,24 @@ def test_oracle_vs_reference_fixture(oracle):
             assert oracle.sign(msg, pk, sk) == sig
 
 
+def test_oracle_vs_torsion_fixture(oracle):
+    """The oracle and the independent big-int closed form against the compiled
+    reference's verdicts on keys A + T, R + T and both, non-canonical keys and
+    R, and S in [2^252, L) (ref_torsion.json, 268 rows, both verdicts present
+    in every torsion class)."""
+    fx = load_json("ref_torsion.json")
+    rows = fx["rows"]
+    assert len(rows) >= 200
+    seen = set()
+    for r in rows:
+        sig, msg, pk = bytes.fromhex(r["sig"]), bytes.fromhex(r["msg"]), bytes.fromhex(r["pk"])
+        assert oracle.verify(sig, msg, pk) == r["verify"], r
+        assert E.verify_rule(sig, msg, pk) == r["verify"], r
+        seen.add((r["class"], r["verify"]))
+    for cls in ("a_plus_t", "mixed"):
+        assert (cls, 0) in seen and (cls, 1) in seen
+
+
 def test_oracle_vs_live_reference(oracle, reflib):
     rng = np.random.default_rng(13)
     for k in range(200):
