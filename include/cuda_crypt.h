@@ -165,13 +165,16 @@ int spl_verify_engine_use(spl_verify_engine *engine, int index);
  * Extension: serving loop (no counterpart in the reference, whose callers
  * make one synchronous ed25519_verify_many / poh_verify_many call per batch).
  * A server owns a device ring of hbm_budget_bytes (0: 1 GiB, at most 4 GiB)
- * with a pinned host mirror, verify_streams + poh_streams streams (0: 6 and 2;
+ * with a pinned host mirror, verify_streams + poh_streams streams (0: 3 and 1;
  * at most 16 together), and one loop thread.  Requests are submitted from any
  * thread and return a ticket (> 0) at once; the loop admits pending requests
  * in arrival order into ring space as it frees up and, whenever a stream is
  * idle, launches every admitted request of that kind as one batch (up to
  * max_batch_signatures signatures or chains; 0: 65,536), so batches on
- * different streams overlap on the device.
+ * different streams overlap on the device.  The streams are plain ones and
+ * share the process's hardware queues; PoH batches are hashed 256 hashes per
+ * chain per launch, one launch in flight per stream, so a verify batch
+ * queued behind PoH work waits for at most one such launch.
  *
  * spl_server_submit_verify takes one request in the ed25519_verify_many
  * layout, with offsets relative to `packets` (packets_size bytes); a signature

@@ -30,16 +30,26 @@
 
 namespace spl {
 
+// Hashes of a chain with `count` hashes in all that fall in the window
+// [skip, skip + limit) of one launch: a chain is hashed in slices by
+// successive launches with skip = 0, limit, 2 limit, ... (the state stays in
+// place between them); skip = 0, limit = all hashes is the whole chain.
+__device__ __forceinline__ uint64_t slice_count(uint64_t count, uint64_t skip, uint64_t limit) {
+    const uint64_t left = count > skip ? count - skip : 0;
+    return left < limit ? left : limit;
+}
+
 __global__ void __launch_bounds__(256) poh_chain_kernel(uint32_t *__restrict__ hashes,
                                                         const uint64_t *__restrict__ counts,
-                                                        uint64_t n) {
+                                                        uint64_t n, uint64_t skip,
+                                                        uint64_t limit) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t *h = hashes + 8 * i;
     uint32_t st[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) st[k] = bswap32(h[k]);
-    uint64_t reps = counts[i];
+    uint64_t reps = slice_count(counts[i], skip, limit);
     for (uint64_t r = 0; r < reps; r++) sha256_32byte(st);
 #pragma unroll
     for (int k = 0; k < 8; k++) h[k] = bswap32(st[k]);
@@ -98,7 +108,8 @@ static_assert(kPohPhase1Rounds >= 9 && kPohPhase1Rounds <= 16,
               "rounds 0..8 (state words and the padding bit) always run in phase 1");
 
 __global__ void __launch_bounds__(kPohSplitThreads)
-poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ counts, uint64_t n) {
+poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ counts, uint64_t n,
+                 uint64_t skip, uint64_t limit) {
     __shared__ uint32_t kw[48][64];              // K[t] + W[t], t = 16..63
     __shared__ uint32_t msg[8][64];              // message words of the coming hash
     __shared__ unsigned long long reps_max;
@@ -108,7 +119,7 @@ poh_split_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ cou
     const bool rounds = threadIdx.x < 64;        // wave-uniform role
     const uint64_t i = (uint64_t)blockIdx.x * 64 + lane;
     const bool valid = i < n;
-    const uint64_t cnt = valid ? counts[i] : 0;
+    const uint64_t cnt = valid ? slice_count(counts[i], skip, limit) : 0;
     if (threadIdx.x == 0) reps_max = 0;
     __syncthreads();
     if (rounds) atomicMax(&reps_max, (unsigned long long)cnt);
@@ -211,15 +222,15 @@ static bool use_split_kernel(uint64_t n) {
 }
 
 hipError_t launch_poh(uint32_t *d_hashes, const uint64_t *d_counts, uint64_t n,
-                      hipStream_t stream) {
-    if (n == 0) return hipSuccess;
+                      hipStream_t stream, uint64_t skip, uint64_t limit) {
+    if (n == 0 || limit == 0) return hipSuccess;
     const uint64_t groups = (n + 63) / 64;
     if (use_split_kernel(n))
         hipLaunchKernelGGL(poh_split_kernel, dim3((uint32_t)groups), dim3(kPohSplitThreads), 0,
-                           stream, d_hashes, d_counts, n);
+                           stream, d_hashes, d_counts, n, skip, limit);
     else
         hipLaunchKernelGGL(poh_chain_kernel, dim3((uint32_t)groups), dim3(64), 0, stream,
-                           d_hashes, d_counts, n);
+                           d_hashes, d_counts, n, skip, limit);
     return hipGetLastError();
 }
 

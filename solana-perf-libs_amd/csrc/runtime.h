@@ -123,11 +123,14 @@ hipError_t stage_elems(uint8_t *h, uint8_t *d, const gpu_Elems *elems, uint32_t 
 // Compute units of a HIP device (256 when the query fails).
 int device_cus(int dev);
 
-// Enqueues SHA-256 chains: hashes[8*i..8*i+8) <- SHA-256^counts[i] of itself,
-// in place, for i < n (poh_verify.hip).  Few chains take the two-waves-per-64
-// kernel, many the one-lane-per-chain kernel.
+// Enqueues SHA-256 chains: hashes[8*i..8*i+8) <- SHA-256^m of itself, in
+// place, for i < n, where m is the part of counts[i] that falls in
+// [skip, skip + limit) (poh_verify.hip).  The defaults hash whole chains;
+// successive launches with skip = 0, limit, 2 limit, ... hash them in slices.
+// Few chains take the two-waves-per-64 kernel, many the one-lane-per-chain
+// kernel.
 hipError_t launch_poh(uint32_t *d_hashes, const uint64_t *d_counts, uint64_t n,
-                      hipStream_t stream);
+                      hipStream_t stream, uint64_t skip = 0, uint64_t limit = ~0ull);
 
 // Frees every cached device/pinned buffer (ed25519_free_gpu_mem).
 void pool_free_buffers();

@@ -110,11 +110,12 @@ class ServedTimeout(RuntimeError):
     """A request did not complete within the wait limit."""
 
 
-def run_served(server, trace, lead_s=0.05, wait_ms=60000):
+def run_served(server, trace, lead_s=0.05, wait_ms=60000, keep_results=False):
     """Submits every request at its arrival time (open loop) and waits for all
     of them.  Returns per-request records: kind, units, latency from the
     scheduled arrival to completion (ms), submit lateness (ms), and whether the
-    result equals the expected one."""
+    result equals the expected one (with keep_results, also the result and
+    the request itself)."""
     # convert every request's arguments before the clock starts, so that a
     # submission at its arrival time is one C call
     prepared = []
@@ -162,8 +163,12 @@ def run_served(server, trace, lead_s=0.05, wait_ms=60000):
             exp = req[2]
             ok = rc == 0 and np.array_equal(res, exp)
             units = exp.size // 32
-        recs.append({"kind": kind, "units": units, "latency_ms": (td - target) / 1e6,
-                     "late_ms": (ts - target) / 1e6, "done_ns": td, "ok": bool(ok)})
+        rec = {"kind": kind, "units": units, "latency_ms": (td - target) / 1e6,
+               "late_ms": (ts - target) / 1e6, "done_ns": td, "ok": bool(ok)}
+        if keep_results:
+            rec["result"] = res[:exp.size].copy() if kind == "verify" else res.copy()
+            rec["request"] = req
+        recs.append(rec)
     span_s = (max(r["done_ns"] for r in recs) - t0) / 1e9 if recs else 0.0
     return recs, span_s, (t_last - t0) / 1e9
 

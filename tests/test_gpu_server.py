@@ -14,9 +14,21 @@ import loadgen
 import perf_libs
 
 # A stall in these tests fails the test after 150 s, with every thread's
-# stack printed, instead of holding the run: the serving loop twice went
-# silent on the GPU with no cause found by reading it.
+# stack printed, instead of holding the run.
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(150, method="thread")]
+
+# The three tests skipped with this reason went silent on the GPU under an
+# earlier serving loop, whose slots were CU-masked streams, each a hardware
+# queue of its own.  Since then the scheduling core has been run under
+# ThreadSanitizer through thousands of server lifetimes with no race and no
+# wrong result (tests/test_server_core.py), and the server creates no
+# hardware queue (plain streams, pooled, PoH in bounded slices); no cause of
+# the stalls was shown, so these cases stay off the GPU.
+# test_config5_served_mix checks their property (each result the same alone
+# and in the mix) on the current server.
+STALLED = ("went silent on the GPU under the earlier serving loop (CU-masked streams); "
+           "no cause shown, so kept off the GPU; its property is checked by "
+           "test_config5_served_mix")
 
 
 def _step(msg):
@@ -92,8 +104,7 @@ def _wait_ok(server, t):
     return rc, times
 
 
-@pytest.mark.skip(reason="hung once on the GPU (silent >180 s) with no cause found by "
-                         "reading the code; kept off the GPU until a cause is found")
+@pytest.mark.skip(reason=STALLED)
 def test_alone_equals_mix(gpu_lib, pools):
     reqs = _requests(pools, seed=1, n_verify=48, n_poh=10)
     server = _server(gpu_lib, hbm_budget_bytes=64 << 20)
@@ -171,8 +182,7 @@ def test_small_ring_backpressure(gpu_lib, pools):
         server.close()
 
 
-@pytest.mark.skip(reason="went silent for >180 s on the GPU (the run was killed) with no cause "
-                         "found by reading the code; kept off the GPU until a cause is found")
+@pytest.mark.skip(reason=STALLED)
 def test_edges(gpu_lib, pools):
     server = _server(gpu_lib, hbm_budget_bytes=8 << 20)
     try:
@@ -230,9 +240,7 @@ def test_poisson_trace(gpu_lib, pools):
     assert s["verify"]["requests"] > 10
 
 
-@pytest.mark.skip(reason="spl_server_destroy went silent for >150 s on the GPU after every request "
-                         "had completed; no cause found by reading the code; kept off the GPU "
-                         "until a cause is found")
+@pytest.mark.skip(reason=STALLED)
 def test_mix_equals_plain_calls(gpu_lib, pools):
     """Requests submitted together (batched by the server, overlapping on the
     device) give the same bytes as each request alone through the drop-in
@@ -261,3 +269,76 @@ def test_mix_equals_plain_calls(gpu_lib, pools):
             assert gpu_lib.poh_verify_many(alone, counts) == 0
         assert np.array_equal(alone, exp), kind
         assert np.array_equal(m, exp), kind
+
+
+def test_config5_served_mix(gpu_lib, oracle):
+    """The served mixed load at its stated shape: Poisson verify requests of
+    log-uniform 1-2048 packets in the validator layout (1-4 signatures per
+    packet, 5 % flipped, 1 % S + L, 1 % small-order keys) and PoH requests of
+    log-uniform 1-512 chains x 12,500 hashes, at two offered loads up to
+    2e6 signatures/s, through one server with its default shape (1 GiB ring).
+    Every result equals the oracle's, and equals the same request alone
+    through the drop-in ed25519_verify_many / poh_verify_many."""
+    def expected(b, poh_in, counts):
+        mask = oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off, threads=8)
+        out = poh_in.copy()
+        oracle.poh_verify_many(out, counts, threads=8)
+        return mask, out
+
+    pools = loadgen.build_pools(gpu_lib, seed=31, num_batches=128, poh_chains=512,
+                                poh_hashes=12500, expected=expected)
+    print(f"config-5 pools: {pools.b.n} signatures in {pools.num_packets} packets, "
+          f"{pools.num_chains} chains x 12,500", flush=True)
+    rng = np.random.default_rng(51)
+    server = _server(gpu_lib)
+    steps = []
+    try:
+        for rate, seconds in ((1e5, 1.0), (2e6, 0.5)):
+            trace = loadgen.make_trace(pools, rng, rate, seconds, poh_rate=20, max_packets=2048,
+                                       max_chains=512)
+            inputs = [req[0].copy() if kind == "poh" else None for _, kind, req in trace]
+            _step(f"offered {rate:.0e} sig/s: {len(trace)} requests")
+            try:
+                recs, span, _ = loadgen.run_served(server, trace, wait_ms=WAIT_MS,
+                                                   keep_results=True)
+            except loadgen.ServedTimeout as exc:
+                server.abandon()
+                pytest.fail(str(exc))
+            s = loadgen.summarize(recs, span)
+            print(f"offered {rate:.0e}:", s, server.stats(reset=True), flush=True)
+            assert s["verify"]["all_exact"] and s["poh"]["all_exact"], s
+            assert s["verify"]["requests"] >= 20 and s["poh"]["requests"] >= 5
+            steps.append((recs, inputs))
+        # chains whose counts straddle the 256-hash launch slices, and a
+        # signature whose message runs past its request's packets (rejected
+        # alone; the others keep their verdicts)
+        counts = np.array([0, 1, 255, 256, 257, 511, 512, 513, 12500, 3], np.uint64)
+        h0 = np.random.default_rng(52).integers(0, 256, 32 * counts.size, dtype=np.uint8)
+        want_h = h0.copy()
+        oracle.poh_verify_many(want_h, counts)
+        h = h0.copy()
+        t_poh = server.submit_poh(h, counts)
+        packets, lens, pk, sig, msg, exp = pools.verify_request(40, 30)
+        lens2 = lens.copy()
+        lens2[2] = packets.size - msg[2] + 1
+        t_v, out = server.submit_verify(packets, lens2, pk, sig, msg)
+        assert t_poh > 0 and t_v > 0
+        assert _wait_ok(server, t_poh)[0] == 0 and _wait_ok(server, t_v)[0] == 0
+        assert np.array_equal(h, want_h)
+        want_v = exp.copy()
+        want_v[2] = 0
+        assert np.array_equal(out[:want_v.size], want_v)
+    finally:
+        _step("spl_server_destroy")
+        server.close()
+    # each request alone through the drop-in calls gives the bytes it got in the mix
+    for recs, inputs in steps:
+        for rec, inp in zip(recs, inputs):
+            req = rec["request"]
+            if rec["kind"] == "verify":
+                packets, lens, pk, sig, msg, exp = req
+                alone = gpu_lib.verify_many([packets], 1232, lens, pk, sig, msg)
+            else:
+                alone = inp.copy()
+                assert gpu_lib.poh_verify_many(alone, req[1]) == 0
+            assert np.array_equal(alone, rec["result"]), rec["kind"]
