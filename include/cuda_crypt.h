@@ -126,6 +126,23 @@ int ed25519_sign_many_device(const uint8_t *d_packets, uint64_t packets_size,
 int poh_verify_many_device(uint8_t *d_hashes, const uint64_t *d_num_hashes, size_t num_elems,
                            void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ed25519_verify_many_device with the same arguments,
+ * kernels and results, whose per-signature table scratch is one fixed device
+ * array of 45 MiB per GPU (held from library load to process end) instead of
+ * a buffer the library allocates.  At most 16,384 signatures per call
+ * (more: returns 2, nothing enqueued).  Makes no allocating call; requires
+ * ed25519_init() first (else returns 1).  Every call on one GPU shares the
+ * array, so calls whose work overlaps on the device race: order them, e.g.
+ * on one stream. */
+int ed25519_verify_many_device_noalloc(const uint8_t *d_packets, uint64_t packets_size,
+                                       const uint32_t *d_message_lens,
+                                       const uint32_t *d_public_key_offsets,
+                                       const uint32_t *d_signature_offsets,
+                                       const uint32_t *d_message_start_offsets,
+                                       uint32_t total_signatures, uint8_t *d_out, void *stream);
+
 /* ------------------------------------------------------------------------
  * Extension: verify engine (no counterpart in the reference).
  * A verify engine owns device buffers for up to max_signatures signatures and

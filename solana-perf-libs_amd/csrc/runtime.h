@@ -72,6 +72,7 @@ struct QueueCtx {
 struct DeviceCtx {
     int id = -1;
     ed25519_tables *d_tables = nullptr;
+    void *noalloc_scratch = nullptr;   // device address of verify_noalloc.hip's array
     QueueCtx q[kQueuesPerDevice];
 };
 
@@ -95,6 +96,15 @@ void release_queue(const Lease &l);
 // Device copy of the tables for HIP device `hip_device` (nullptr if that
 // device is not in the pool).
 const ed25519_tables *device_tables(int hip_device);
+
+// Without initialising anything: true when the pool is set up and holds HIP
+// device `hip_device`, with its tables and the address of the fixed verify
+// scratch (nullptr if that address could not be taken).
+bool pool_device_ready(int hip_device, const ed25519_tables **tables, void **noalloc_scratch);
+
+// Address of verify_noalloc.hip's fixed scratch array on the current device
+// (called once per device by the pool's initialisation).
+hipError_t noalloc_bind_device(void **ptr);
 
 // Device scratch for launches enqueued by the device-resident entry points:
 // one grow-only buffer per (device, stream), so launches on one stream reuse

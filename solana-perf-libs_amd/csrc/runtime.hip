@@ -215,6 +215,17 @@ static int pool_init_locked() {
         SPL_HIP_TRY(hipSetDevice(d));
         SPL_HIP_TRY(hipMalloc(&dc.d_tables, sizeof(ed25519_tables)));
         SPL_HIP_TRY(hipMemcpy(dc.d_tables, &T, sizeof(ed25519_tables), hipMemcpyHostToDevice));
+        {
+            // the fixed scratch of the allocation-free verify entry point;
+            // the device stays usable for every other entry point without it
+            const hipError_t e = noalloc_bind_device(&dc.noalloc_scratch);
+            if (e != hipSuccess) {
+                fprintf(stderr, "libcuda-crypt: device %d: no address for the fixed verify "
+                                "scratch: %s\n", d, hipGetErrorString(e));
+                dc.noalloc_scratch = nullptr;
+                (void)hipGetLastError();
+            }
+        }
         for (int q = 0; q < kQueuesPerDevice; q++) {
             SPL_HIP_TRY(hipStreamCreate(&dc.q[q].stream));
             dc.q[q].h_stage.host = true;
@@ -265,6 +276,17 @@ const ed25519_tables *device_tables(int hip_device) {
     for (int d = 0; d < g_num_devices; d++)
         if (g_dev[d].id == hip_device) return g_dev[d].d_tables;
     return nullptr;
+}
+
+bool pool_device_ready(int hip_device, const ed25519_tables **tables, void **noalloc_scratch) {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    for (int d = 0; d < g_num_devices; d++)
+        if (g_dev[d].id == hip_device) {
+            *tables = g_dev[d].d_tables;
+            *noalloc_scratch = g_dev[d].noalloc_scratch;
+            return true;
+        }
+    return false;
 }
 
 struct StreamScratch {

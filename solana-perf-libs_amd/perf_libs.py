@@ -32,6 +32,7 @@ EXPORTS = (
     "ed25519_get_checked_scalar", "ed25519_check_packed_ge_small_order",
     "ed25519_free_gpu_mem", "poh_verify_many_set_verbose", "poh_verify_many",
     "ed25519_verify_many_device", "ed25519_sign_many_device", "poh_verify_many_device",
+    "ed25519_verify_many_device_noalloc",
     "spl_verify_engine_create", "spl_verify_engine_destroy", "spl_verify_engine_buffers",
     "spl_verify_engine_run", "spl_verify_engine_sync", "spl_verify_engine_autotune",
     "spl_verify_engine_config", "spl_verify_engine_candidates", "spl_verify_engine_use",
@@ -90,6 +91,8 @@ class PerfLibs:
         lib.ed25519_verify_many_device.restype = ctypes.c_int
         lib.ed25519_sign_many_device.argtypes = dev
         lib.ed25519_sign_many_device.restype = ctypes.c_int
+        lib.ed25519_verify_many_device_noalloc.argtypes = dev
+        lib.ed25519_verify_many_device_noalloc.restype = ctypes.c_int
         lib.poh_verify_many_device.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
         lib.poh_verify_many_device.restype = ctypes.c_int
         lib.spl_verify_engine_create.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
@@ -243,6 +246,13 @@ class PerfLibs:
                            stream=0):
         return self.lib.ed25519_verify_many_device(d_packets, packets_size, d_lens, d_pk, d_sig,
                                                    d_msg, n, d_out, stream)
+
+    def verify_many_device_noalloc(self, d_packets, packets_size, d_lens, d_pk, d_sig, d_msg, n,
+                                   d_out, stream=0):
+        """ed25519_verify_many_device with the library's fixed scratch (no
+        allocation; at most 16,384 signatures; needs init() first)."""
+        return self.lib.ed25519_verify_many_device_noalloc(d_packets, packets_size, d_lens, d_pk,
+                                                           d_sig, d_msg, n, d_out, stream)
 
     def sign_many_device(self, d_packets, packets_size, d_lens, d_pk, d_sk, d_msg, n, d_out,
                          stream=0):

@@ -15,10 +15,15 @@ def pytest_configure(config):
 
 
 def pytest_collection_modifyitems(session, config, items):
-    """The serving-loop tests run last: each one starts a loop thread and
-    hardware queues of its own, and a stall there must not keep the kernel
-    parity tests from running."""
-    items.sort(key=lambda it: it.nodeid.split("::")[0].endswith("test_gpu_server.py"))
+    """The serving-loop tests run after the kernel parity tests (each starts a
+    loop thread, and a stall there must not keep those from running), and the
+    allocation-free entry point's test runs last of all."""
+    def rank(it):
+        path = it.nodeid.split("::")[0]
+        if path.endswith("test_zz_no_alloc.py"):
+            return 2
+        return 1 if path.endswith("test_gpu_server.py") else 0
+    items.sort(key=rank)
 
 
 @pytest.fixture(scope="session")
