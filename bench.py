@@ -179,64 +179,56 @@ class Watchdog:
                 os._exit(3)
 
 
-def in_flight_leg(lib, b, eng, mask, steps, depths=(2, 3)):
-    """The same validator batch with several batches in flight: D engines,
-    each with its own buffers, stream and instantiated graph, steps issued
-    round-robin so up to D launches overlap (as concurrent callers of the
-    reference's ed25519_verify_many overlap on its per-context streams).  One
-    launch leaves a third of the SIMDs idle (651 waves on 1,024 SIMDs);
-    overlapping launches fill them.  Timed in the tuned configuration and in
-    the one-shot half-scalar ones: a persistent grid claims every SIMD when it
-    launches, so one-shot launches overlap better.  Reported beside `value`,
-    which times one launch per step."""
+def in_flight_leg(lib, b, mask, steps, depths=(2, 3)):
+    """The same validator batch with batches in flight inside one engine: an
+    engine of D buffer sets (each with its own buffers, stream and graph),
+    the batch loaded into every set, autotuned with every set in flight, then
+    `steps` batches issued through next()/run_set(), so up to D launches
+    overlap.  One launch leaves a third of the SIMDs idle (651 waves on 1,024
+    SIMDs); the next set's launch fills them.  Whole-job sig-verifies/s over
+    the steps, wall clock between synchronisations; every set's mask checked
+    against the one-launch-per-step mask.  Reported beside `value`, which
+    times one launch per step."""
     import hip_rt
     import perf_libs
-    engs = []
-    out = {"note": "whole-job sig-verifies/s over %d batches, D engines round-robin; every "
-                   "engine's mask checked against the tuned engine's" % steps, "configs": {}}
-    try:
-        for _ in range(max(depths)):
-            e = perf_libs.VerifyEngine(lib, b.n, b.packets.size)
-            engs.append(e)
-            hip_rt.upload_to(e.d_packets, b.packets)
-            for dptr, tab in ((e.d_lens, b.lens), (e.d_pk, b.pk_off), (e.d_sig, b.sig_off),
-                              (e.d_msg, b.msg_off)):
-                hip_rt.upload_to(dptr, tab)
-        cands = engs[0].candidates()
-        configs = [eng.config()] + ["one-shot W=4 scalars=half waves_per_eu=1",
-                                    "one-shot W=5 scalars=half waves_per_eu=1"]
-        best = None
-        for cfg in dict.fromkeys(c for c in configs if c in cands):
-            for e in engs:
-                e.use(cands.index(cfg))
-            row = {}
-            for d in depths:
-                use = engs[:d]
-                for e in use:                        # warm-up: one launch each
-                    assert e.run(b.n) == 0
-                for e in use:
-                    e.sync()
-                hip_rt.synchronize()
-                t0 = time.perf_counter()
-                for k in range(steps):
-                    assert use[k % d].run(b.n) == 0
-                for e in use:
-                    e.sync()
-                hip_rt.synchronize()
-                wall = time.perf_counter() - t0
-                exact = all(np.array_equal(hip_rt.download_from(e.d_out, b.n), mask)
-                            for e in use)
-                rate = steps * b.n / wall
-                row[f"depth_{d}"] = {"sig_verifies_per_s": rate,
-                                     "ms_per_batch": wall * 1e3 / steps, "masks_equal": exact}
-                note(f"bench: {d} batches in flight ({cfg}): {rate:.3e} sig/s, exact {exact}")
-                if exact and (best is None or rate > best["sig_verifies_per_s"]):
-                    best = {"config": cfg, "depth": d, "sig_verifies_per_s": rate}
-            out["configs"][cfg] = row
-        out["best"] = best
-    finally:
-        for e in engs:
-            e.close()
+    out = {"note": "whole-job sig-verifies/s over %d batches through one engine of D buffer "
+                   "sets (next()/run_set()), autotuned with D in flight; every set's mask "
+                   "checked" % steps, "depths": {}}
+    best = None
+    for d in depths:
+        eng = perf_libs.VerifyEngine(lib, b.n, b.packets.size, depth=d)
+        try:
+            for k in range(d):
+                bufs = eng.set_buffers(k)
+                for dptr, tab in zip(bufs[:5], (b.packets, b.lens, b.pk_off, b.sig_off,
+                                                b.msg_off)):
+                    hip_rt.upload_to(dptr, tab)
+            report = eng.autotune(b.n)
+            note(f"bench: autotune with {d} in flight\n" + report)
+            for _ in range(d):                       # warm-up: one batch per set
+                assert eng.run_set(eng.next(), b.n) == 0
+            assert eng.sync() == 0
+            hip_rt.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                assert eng.run_set(eng.next(), b.n) == 0
+            assert eng.sync() == 0
+            hip_rt.synchronize()
+            wall = time.perf_counter() - t0
+            exact = all(np.array_equal(hip_rt.download_from(eng.set_buffers(k)[5], b.n), mask)
+                        for k in range(d))
+            rate = steps * b.n / wall
+            row = {"config": eng.config(), "sig_verifies_per_s": rate,
+                   "ms_per_batch": wall * 1e3 / steps, "masks_equal": exact}
+            out["depths"][str(d)] = row
+            log(f"bench: {d} batches in flight in one engine ({row['config']}): "
+                f"{rate:.3e} sig/s, exact {exact}")
+            if exact and (best is None or rate > best["sig_verifies_per_s"]):
+                best = {"config": row["config"], "depth": d, "sig_verifies_per_s": rate,
+                        "masks_equal": exact}
+        finally:
+            eng.close()
+    out["best"] = best
     return out
 
 
@@ -621,7 +613,7 @@ def main():
     try:
         # 60 batches per depth (~40 ms): enough that filling and draining the
         # pipeline at the ends does not weigh on the rate
-        result["in_flight"] = in_flight_leg(lib, b, eng, mask, max(args.steps, 60))
+        result["in_flight"] = in_flight_leg(lib, b, mask, max(args.steps, 60))
     except Exception as exc:                     # reported, never fatal to the line
         log(f"bench: in-flight leg failed: {exc!r}")
         result["in_flight"] = {"error": repr(exc)}

@@ -159,10 +159,33 @@ int ed25519_verify_many_device_noalloc(const uint8_t *d_packets, uint64_t packet
  * the buffers, keeps the fastest and records it
  * for that batch-size class on the device, where the ABI calls pick it up.
  * An engine is used by one thread at a time.  Functions returning int: 0 ok,
- * 2 bad arguments, 3 device error. */
+ * 2 bad arguments, 3 device error.
+ *
+ * Batches in flight: spl_verify_engine_create_pipelined makes an engine of
+ * `depth` (1..8) buffer sets, each with its own buffers, stream and graph, so
+ * consecutive batches overlap on the device (one validator-sized launch
+ * leaves a third of the SIMDs idle; the next set's launch fills them).
+ * spl_verify_engine_next waits until the next set in round-robin order has
+ * finished its previous batch and returns its index; the caller fills that
+ * set's buffers (spl_verify_engine_set_buffers), starts it with
+ * spl_verify_engine_run_set, and reads its mask after
+ * spl_verify_engine_sync_set (or the next time next() hands the set out).
+ * The single-set calls (buffers, run) address set 0, and sync waits for
+ * every set.  Autotuning an engine of several sets times each configuration
+ * with every set in flight (load the batch into each set first) and records
+ * nothing for the ABI calls. */
 typedef struct spl_verify_engine spl_verify_engine;
 spl_verify_engine *spl_verify_engine_create(uint32_t max_signatures, uint64_t max_packet_bytes);
+spl_verify_engine *spl_verify_engine_create_pipelined(uint32_t max_signatures,
+                                                      uint64_t max_packet_bytes, uint32_t depth);
 void spl_verify_engine_destroy(spl_verify_engine *engine);
+int spl_verify_engine_depth(spl_verify_engine *engine);
+int spl_verify_engine_set_buffers(spl_verify_engine *engine, uint32_t set, uint8_t **d_packets,
+                                  uint32_t **d_lens, uint32_t **d_pk_off, uint32_t **d_sig_off,
+                                  uint32_t **d_msg_off, uint8_t **d_out, void **stream);
+int spl_verify_engine_next(spl_verify_engine *engine);
+int spl_verify_engine_run_set(spl_verify_engine *engine, uint32_t set, uint32_t n);
+int spl_verify_engine_sync_set(spl_verify_engine *engine, uint32_t set);
 int spl_verify_engine_buffers(spl_verify_engine *engine, uint8_t **d_packets, uint32_t **d_lens,
                               uint32_t **d_public_key_offsets, uint32_t **d_signature_offsets,
                               uint32_t **d_message_start_offsets, uint8_t **d_out, void **stream);

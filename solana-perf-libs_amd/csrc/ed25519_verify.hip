@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "verify_kernels.h"
 #include "../../include/cuda_crypt.h"
@@ -102,58 +103,114 @@ VerifyJob make_verify_job(const uint8_t *packets, uint64_t size, const uint32_t 
 
 // ------------------------------------------------------------ verify engine
 
-// Device buffers sized for a capacity, one stream, and one instantiated graph
-// of the tuned launch sequence.  Inputs are written by the caller straight
-// into the engine's buffers; run() sets the signature count in device memory
-// (outside the graph) and replays the graph.
+// Device buffers sized for a capacity and one instantiated graph of the tuned
+// launch sequence, in `depth` buffer sets.  Each set has its own stream,
+// buffers, scratch, work counters and graph, so batches in different sets run
+// concurrently: one launch of a validator batch leaves a third of the SIMDs
+// idle (651 waves of 64 signatures on 1,024 SIMDs), and the next set's launch
+// fills them.  Inputs are written by the caller straight into a set's
+// buffers; run() sets the signature count in device memory (outside the
+// graph) and replays the set's graph.
 struct spl_verify_engine {
+    struct Set {
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;           // recorded after each run of the set
+        bool ran = false;
+        spl::Buf packets, tabs, out, scratch, work, order, n_dev;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+
     int dev = 0;
     int cus = 256;
-    hipStream_t stream = nullptr;
     uint32_t cap_sigs = 0;
     uint64_t cap_bytes = 0;
-    spl::Buf packets, tabs, out, scratch, work, order, n_dev;
+    std::vector<Set> sets;
+    uint32_t next_set = 0;
     spl::VerifyConfig cfg = spl::kDefaultVerifyConfig;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     const spl::ed25519_tables *T = nullptr;
 
-    spl::VerifyJob job() const {
-        const uint32_t *t = tabs.as<uint32_t>();
-        spl::VerifyJob j = spl::make_verify_job(packets.as<uint8_t>(), cap_bytes, t,
+    spl::VerifyJob job(const Set &s) const {
+        const uint32_t *t = s.tabs.as<uint32_t>();
+        spl::VerifyJob j = spl::make_verify_job(s.packets.as<uint8_t>(), cap_bytes, t,
                                                 t + cap_sigs, t + 2ull * cap_sigs,
-                                                t + 3ull * cap_sigs, out.as<uint8_t>(),
-                                                scratch.ptr, T);
-        j.n_dev = n_dev.as<uint32_t>();
-        j.work = work.as<uint32_t>();
-        j.order = order.as<uint32_t>();
+                                                t + 3ull * cap_sigs, s.out.as<uint8_t>(),
+                                                s.scratch.ptr, T);
+        j.n_dev = s.n_dev.as<uint32_t>();
+        j.work = s.work.as<uint32_t>();
+        j.order = s.order.as<uint32_t>();
         return j;
     }
 
-    hipError_t ensure_scratch(const spl::VerifyConfig &c) {
-        return scratch.reserve(spl::verify_scratch_bytes(c, cap_sigs, cus));
+    hipError_t alloc_set(Set &s) {
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess ||
+            (e = s.packets.reserve(cap_bytes + 16)) != hipSuccess ||
+            (e = s.tabs.reserve(16ull * cap_sigs)) != hipSuccess ||
+            (e = s.out.reserve(cap_sigs)) != hipSuccess ||
+            (e = s.work.reserve(64 * 4)) != hipSuccess ||
+            (e = s.order.reserve(4ull * cap_sigs)) != hipSuccess ||
+            (e = s.n_dev.reserve(4)) != hipSuccess)
+            return e;
+        return hipSuccess;
     }
 
-    void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+    void free_set(Set &s) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        drop_graph(s);
+        s.packets.release();
+        s.tabs.release();
+        s.out.release();
+        s.scratch.release();
+        s.work.release();
+        s.order.release();
+        s.n_dev.release();
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+        s = Set{};
     }
 
-    // Captures the launch sequence of `cfg` (grid sized for the capacity; the
-    // kernels read the count from n_dev).
-    hipError_t capture() {
-        drop_graph();
-        hipError_t e = ensure_scratch(cfg);
+    static void drop_graph(Set &s) {
+        if (s.exec) (void)hipGraphExecDestroy(s.exec);
+        if (s.graph) (void)hipGraphDestroy(s.graph);
+        s.exec = nullptr;
+        s.graph = nullptr;
+    }
+
+    // Captures the launch sequence of `cfg` for one set (grid sized for the
+    // capacity; the kernels read the count from the set's n_dev).
+    hipError_t capture(Set &s) {
+        drop_graph(s);
+        hipError_t e = s.scratch.reserve(spl::verify_scratch_bytes(cfg, cap_sigs, cus));
         if (e != hipSuccess) return e;
-        e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        e = hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
-        hipError_t le = spl::launch_verify_cfg(cfg, job(), cap_sigs, cus, stream);
-        e = hipStreamEndCapture(stream, &graph);
+        hipError_t le = spl::launch_verify_cfg(cfg, job(s), cap_sigs, cus, s.stream);
+        e = hipStreamEndCapture(s.stream, &s.graph);
         if (le != hipSuccess) return le;
         if (e != hipSuccess) return e;
-        return hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        return hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0);
+    }
+
+    hipError_t capture_all() {
+        for (Set &s : sets) {
+            if (s.stream) {
+                const hipError_t e = hipStreamSynchronize(s.stream);
+                if (e != hipSuccess) return e;
+            }
+            const hipError_t e = capture(s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+
+    hipError_t run(Set &s, uint32_t n) {
+        spl::launch_set_n(s.n_dev.as<uint32_t>(), n, s.stream);
+        hipError_t e = hipGraphLaunch(s.exec, s.stream);
+        if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+        if (e == hipSuccess) s.ran = true;
+        return e;
     }
 };
 
@@ -236,8 +293,9 @@ int ed25519_verify_many_device(const uint8_t *d_packets, uint64_t packets_size,
     return 0;
 }
 
-spl_verify_engine *spl_verify_engine_create(uint32_t max_signatures, uint64_t max_packet_bytes) {
-    if (max_signatures == 0 || max_packet_bytes == 0) return nullptr;
+spl_verify_engine *spl_verify_engine_create_pipelined(uint32_t max_signatures,
+                                                      uint64_t max_packet_bytes, uint32_t depth) {
+    if (max_signatures == 0 || max_packet_bytes == 0 || depth == 0 || depth > 8) return nullptr;
     if (pool_init() <= 0) return nullptr;
     auto *eng = new spl_verify_engine();
     bool ok = hipGetDevice(&eng->dev) == hipSuccess;
@@ -247,20 +305,15 @@ spl_verify_engine *spl_verify_engine_create(uint32_t max_signatures, uint64_t ma
         eng->cus = device_cus(eng->dev);
         eng->cap_sigs = max_signatures;
         eng->cap_bytes = max_packet_bytes;
-        ok = hipStreamCreate(&eng->stream) == hipSuccess &&
-             eng->packets.reserve(max_packet_bytes + 16) == hipSuccess &&
-             eng->tabs.reserve(16ull * max_signatures) == hipSuccess &&
-             eng->out.reserve(max_signatures) == hipSuccess &&
-             eng->work.reserve(64 * 4) == hipSuccess &&
-             eng->order.reserve(4ull * max_signatures) == hipSuccess &&
-             eng->n_dev.reserve(4) == hipSuccess;
+        eng->sets.resize(depth);
+        for (auto &st : eng->sets) ok = ok && eng->alloc_set(st) == hipSuccess;
     }
-    if (ok) {
+    if (ok && depth == 1) {
         std::lock_guard<std::mutex> g(g_tune_mu);
         const TuneEntry &t = g_tune[eng->dev][size_class(max_signatures)];
         if (t.valid) eng->cfg = t.best;
     }
-    ok = ok && eng->capture() == hipSuccess;
+    ok = ok && eng->capture_all() == hipSuccess;
     if (!ok) {
         fprintf(stderr, "spl_verify_engine_create: failed\n");
         spl_verify_engine_destroy(eng);
@@ -269,42 +322,55 @@ spl_verify_engine *spl_verify_engine_create(uint32_t max_signatures, uint64_t ma
     return eng;
 }
 
+spl_verify_engine *spl_verify_engine_create(uint32_t max_signatures, uint64_t max_packet_bytes) {
+    return spl_verify_engine_create_pipelined(max_signatures, max_packet_bytes, 1);
+}
+
 void spl_verify_engine_destroy(spl_verify_engine *eng) {
     if (!eng) return;
-    if (eng->stream) (void)hipStreamSynchronize(eng->stream);
-    eng->drop_graph();
-    eng->packets.release();
-    eng->tabs.release();
-    eng->out.release();
-    eng->scratch.release();
-    eng->work.release();
-    eng->order.release();
-    eng->n_dev.release();
-    if (eng->stream) (void)hipStreamDestroy(eng->stream);
+    for (auto &st : eng->sets) eng->free_set(st);
     delete eng;
+}
+
+int spl_verify_engine_depth(spl_verify_engine *eng) { return eng ? (int)eng->sets.size() : -2; }
+
+int spl_verify_engine_set_buffers(spl_verify_engine *eng, uint32_t set, uint8_t **d_packets,
+                                  uint32_t **d_lens, uint32_t **d_pk_off, uint32_t **d_sig_off,
+                                  uint32_t **d_msg_off, uint8_t **d_out, void **stream) {
+    if (!eng || set >= eng->sets.size()) return 2;
+    const spl_verify_engine::Set &st = eng->sets[set];
+    uint32_t *t = st.tabs.as<uint32_t>();
+    if (d_packets) *d_packets = st.packets.as<uint8_t>();
+    if (d_lens) *d_lens = t;
+    if (d_pk_off) *d_pk_off = t + eng->cap_sigs;
+    if (d_sig_off) *d_sig_off = t + 2ull * eng->cap_sigs;
+    if (d_msg_off) *d_msg_off = t + 3ull * eng->cap_sigs;
+    if (d_out) *d_out = st.out.as<uint8_t>();
+    if (stream) *stream = st.stream;
+    return 0;
 }
 
 int spl_verify_engine_buffers(spl_verify_engine *eng, uint8_t **d_packets, uint32_t **d_lens,
                               uint32_t **d_pk_off, uint32_t **d_sig_off, uint32_t **d_msg_off,
                               uint8_t **d_out, void **stream) {
-    if (!eng) return 2;
-    uint32_t *t = eng->tabs.as<uint32_t>();
-    if (d_packets) *d_packets = eng->packets.as<uint8_t>();
-    if (d_lens) *d_lens = t;
-    if (d_pk_off) *d_pk_off = t + eng->cap_sigs;
-    if (d_sig_off) *d_sig_off = t + 2ull * eng->cap_sigs;
-    if (d_msg_off) *d_msg_off = t + 3ull * eng->cap_sigs;
-    if (d_out) *d_out = eng->out.as<uint8_t>();
-    if (stream) *stream = eng->stream;
-    return 0;
+    return spl_verify_engine_set_buffers(eng, 0, d_packets, d_lens, d_pk_off, d_sig_off,
+                                         d_msg_off, d_out, stream);
 }
 
-int spl_verify_engine_run(spl_verify_engine *eng, uint32_t n) {
-    if (!eng) return 2;
+int spl_verify_engine_next(spl_verify_engine *eng) {
+    if (!eng) return -2;
+    const uint32_t k = eng->next_set;
+    spl_verify_engine::Set &st = eng->sets[k];
+    if (st.ran && hipEventSynchronize(st.done) != hipSuccess) return -3;
+    eng->next_set = (k + 1) % (uint32_t)eng->sets.size();
+    return (int)k;
+}
+
+int spl_verify_engine_run_set(spl_verify_engine *eng, uint32_t set, uint32_t n) {
+    if (!eng || set >= eng->sets.size()) return 2;
     if (n > eng->cap_sigs) return 2;
     if (n == 0) return 0;
-    launch_set_n(eng->n_dev.as<uint32_t>(), n, eng->stream);
-    hipError_t e = hipGraphLaunch(eng->exec, eng->stream);
+    const hipError_t e = eng->run(eng->sets[set], n);
     if (e != hipSuccess) {
         fprintf(stderr, "spl_verify_engine_run: %s\n", hipGetErrorString(e));
         return 3;
@@ -312,9 +378,21 @@ int spl_verify_engine_run(spl_verify_engine *eng, uint32_t n) {
     return 0;
 }
 
+int spl_verify_engine_run(spl_verify_engine *eng, uint32_t n) {
+    return spl_verify_engine_run_set(eng, 0, n);
+}
+
+int spl_verify_engine_sync_set(spl_verify_engine *eng, uint32_t set) {
+    if (!eng || set >= eng->sets.size()) return 2;
+    return hipStreamSynchronize(eng->sets[set].stream) == hipSuccess ? 0 : 3;
+}
+
 int spl_verify_engine_sync(spl_verify_engine *eng) {
     if (!eng) return 2;
-    return hipStreamSynchronize(eng->stream) == hipSuccess ? 0 : 3;
+    int rc = 0;
+    for (uint32_t k = 0; k < eng->sets.size(); k++)
+        if (spl_verify_engine_sync_set(eng, k) != 0) rc = 3;
+    return rc;
 }
 
 int spl_verify_engine_config(spl_verify_engine *eng, char *buf, size_t len) {
@@ -346,21 +424,35 @@ int spl_verify_engine_use(spl_verify_engine *eng, int index) {
     const int nc = verify_config_candidates(cand, 64);
     if (index < 0 || index >= nc) return 2;
     eng->cfg = cand[index];
-    return eng->capture() == hipSuccess ? 0 : 3;
+    return eng->capture_all() == hipSuccess ? 0 : 3;
 }
 
 // Times every candidate configuration on the batch currently in the engine's
-// buffers (n signatures), keeps the fastest for this engine (recapturing its
-// graph) and records it for n's size class on this device.
+// buffer sets (n signatures in each), keeps the fastest for this engine
+// (recapturing its graphs) and, for an engine of one set, records it for n's
+// size class on this device.
 int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
                                size_t report_len) {
     if (!eng || n == 0 || n > eng->cap_sigs) return 2;
     VerifyConfig cand[64];
     const int nc = verify_config_candidates(cand, 64);
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return 3;
-    if (hipEventCreate(&e1) != hipSuccess) {
+    const uint32_t depth = (uint32_t)eng->sets.size();
+    hipStream_t s0 = eng->sets[0].stream;
+    hipEvent_t e0, e1, mark[8];
+    int made = 0;
+    bool ev_ok = hipEventCreate(&e0) == hipSuccess;
+    if (ev_ok && hipEventCreate(&e1) != hipSuccess) {
         (void)hipEventDestroy(e0);
+        ev_ok = false;
+    }
+    for (; ev_ok && made < (int)depth; made++)
+        if (hipEventCreateWithFlags(&mark[made], hipEventDisableTiming) != hipSuccess) break;
+    if (!ev_ok || made < (int)depth) {
+        for (int k = 0; k < made; k++) (void)hipEventDestroy(mark[k]);
+        if (ev_ok) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
         return 3;
     }
     float best_ms = 1e30f, best_plain_ms = 1e30f;
@@ -369,33 +461,39 @@ int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
     std::string rep;
     int rc = 0;
     // Each candidate is timed as the engine runs it: its launch sequence
-    // captured and instantiated as the graph, and steps of
-    // spl_verify_engine_run (write n, replay the graph) issued back to back:
-    // one to warm the caches, then four between the events, whose mean is the
-    // candidate's time.  Timing the kernels launched one at a time, or the
-    // best of single replays, favoured the persistent forms (their prepass
-    // kernels are separate graph nodes, and their steps vary more): one tied
-    // the one-shot form there and then ran back to back 1 % slower.
+    // captured and instantiated as every set's graph, and steps (write n,
+    // replay the graph) issued back to back, round-robin over the sets: one
+    // per set to warm the caches, then kTimedSteps per set between the events
+    // (every set's stream starts after e0, and e1 follows the last step of
+    // every set); the time per step is the candidate's time.  Timing the
+    // kernels launched one at a time, or the best of single replays,
+    // favoured the persistent forms (their prepass kernels are separate graph
+    // nodes, and their steps vary more): one tied the one-shot form there and
+    // then ran back to back 1 % slower.
     constexpr int kTimedSteps = 4;
     for (int i = 0; i < nc && rc == 0; i++) {
         const VerifyConfig &c = cand[i];
         eng->cfg = c;
-        if (eng->capture() != hipSuccess) continue;
+        if (eng->capture_all() != hipSuccess) continue;
         float ms = 0;
-        launch_set_n(eng->n_dev.as<uint32_t>(), n, eng->stream);
-        if (hipGraphLaunch(eng->exec, eng->stream) != hipSuccess ||
-            hipStreamSynchronize(eng->stream) != hipSuccess ||
-            hipEventRecord(e0, eng->stream) != hipSuccess)
-            rc = 3;
-        for (int r = 0; r < kTimedSteps && rc == 0; r++) {
-            launch_set_n(eng->n_dev.as<uint32_t>(), n, eng->stream);
-            if (hipGraphLaunch(eng->exec, eng->stream) != hipSuccess) rc = 3;
-        }
-        if (rc == 0 && (hipEventRecord(e1, eng->stream) != hipSuccess ||
+        for (auto &st : eng->sets)
+            if (eng->run(st, n) != hipSuccess) rc = 3;
+        for (auto &st : eng->sets)
+            if (rc == 0 && hipStreamSynchronize(st.stream) != hipSuccess) rc = 3;
+        if (rc == 0 && hipEventRecord(e0, s0) != hipSuccess) rc = 3;
+        for (uint32_t k = 1; k < depth && rc == 0; k++)
+            if (hipStreamWaitEvent(eng->sets[k].stream, e0, 0) != hipSuccess) rc = 3;
+        for (int r = 0; r < kTimedSteps * (int)depth && rc == 0; r++)
+            if (eng->run(eng->sets[r % depth], n) != hipSuccess) rc = 3;
+        for (uint32_t k = 1; k < depth && rc == 0; k++)
+            if (hipEventRecord(mark[k], eng->sets[k].stream) != hipSuccess ||
+                hipStreamWaitEvent(s0, mark[k], 0) != hipSuccess)
+                rc = 3;
+        if (rc == 0 && (hipEventRecord(e1, s0) != hipSuccess ||
                         hipEventSynchronize(e1) != hipSuccess ||
                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = 3;
-        ms /= kTimedSteps;
+        ms /= (float)(kTimedSteps * depth);
         if (rc) break;
         char name[128], line[160];
         config_name(c, name, sizeof name);
@@ -406,14 +504,15 @@ int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    for (int k = 0; k < made; k++) (void)hipEventDestroy(mark[k]);
     if (rc) {
         eng->cfg = before;                     // leave the engine as it was
-        (void)eng->capture();
+        (void)eng->capture_all();
         return rc;
     }
     eng->cfg = best;
-    if (eng->capture() != hipSuccess) return 3;
-    {
+    if (eng->capture_all() != hipSuccess) return 3;
+    if (depth == 1) {
         std::lock_guard<std::mutex> g(g_tune_mu);
         TuneEntry &t = g_tune[eng->dev][size_class(n)];
         t.valid = true;

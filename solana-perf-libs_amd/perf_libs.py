@@ -34,6 +34,9 @@ EXPORTS = (
     "ed25519_verify_many_device", "ed25519_sign_many_device", "poh_verify_many_device",
     "ed25519_verify_many_device_noalloc",
     "spl_verify_engine_create", "spl_verify_engine_destroy", "spl_verify_engine_buffers",
+    "spl_verify_engine_create_pipelined", "spl_verify_engine_depth",
+    "spl_verify_engine_set_buffers", "spl_verify_engine_next", "spl_verify_engine_run_set",
+    "spl_verify_engine_sync_set",
     "spl_verify_engine_run", "spl_verify_engine_sync", "spl_verify_engine_autotune",
     "spl_verify_engine_config", "spl_verify_engine_candidates", "spl_verify_engine_use",
     "spl_server_create", "spl_server_destroy",
@@ -97,8 +100,21 @@ class PerfLibs:
         lib.poh_verify_many_device.restype = ctypes.c_int
         lib.spl_verify_engine_create.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
         lib.spl_verify_engine_create.restype = _vp
+        lib.spl_verify_engine_create_pipelined.argtypes = [ctypes.c_uint32, ctypes.c_uint64,
+                                                           ctypes.c_uint32]
+        lib.spl_verify_engine_create_pipelined.restype = _vp
         lib.spl_verify_engine_destroy.argtypes = [_vp]
         lib.spl_verify_engine_destroy.restype = None
+        lib.spl_verify_engine_depth.argtypes = [_vp]
+        lib.spl_verify_engine_depth.restype = ctypes.c_int
+        lib.spl_verify_engine_set_buffers.argtypes = [_vp, ctypes.c_uint32] + [ctypes.POINTER(_vp)] * 7
+        lib.spl_verify_engine_set_buffers.restype = ctypes.c_int
+        lib.spl_verify_engine_next.argtypes = [_vp]
+        lib.spl_verify_engine_next.restype = ctypes.c_int
+        lib.spl_verify_engine_run_set.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
+        lib.spl_verify_engine_run_set.restype = ctypes.c_int
+        lib.spl_verify_engine_sync_set.argtypes = [_vp, ctypes.c_uint32]
+        lib.spl_verify_engine_sync_set.restype = ctypes.c_int
         lib.spl_verify_engine_buffers.argtypes = [_vp] + [ctypes.POINTER(_vp)] * 7
         lib.spl_verify_engine_buffers.restype = ctypes.c_int
         lib.spl_verify_engine_run.argtypes = [_vp, ctypes.c_uint32]
@@ -280,22 +296,40 @@ class PerfLibs:
 
 
 class VerifyEngine:
-    """spl_verify_engine: device buffers + graph-captured tuned verify launch."""
+    """spl_verify_engine: device buffers + graph-captured tuned verify launch,
+    in `depth` buffer sets (batches in flight).  The d_* attributes are set
+    0's buffers; set_buffers(k) gives set k's."""
 
-    def __init__(self, lib, max_signatures, max_packet_bytes):
+    def __init__(self, lib, max_signatures, max_packet_bytes, depth=1):
         self.lib = lib.lib
-        self.h = self.lib.spl_verify_engine_create(max_signatures, max_packet_bytes)
+        self.h = self.lib.spl_verify_engine_create_pipelined(max_signatures, max_packet_bytes,
+                                                             depth)
         if not self.h:
             raise RuntimeError("spl_verify_engine_create failed")
-        ptrs = [_vp() for _ in range(7)]
-        rc = self.lib.spl_verify_engine_buffers(self.h, *[ctypes.byref(p) for p in ptrs])
-        assert rc == 0
         (self.d_packets, self.d_lens, self.d_pk, self.d_sig, self.d_msg, self.d_out,
-         self.stream) = [p.value for p in ptrs]
+         self.stream) = self.set_buffers(0)
         self.cap = max_signatures
+        self.depth = depth
+
+    def set_buffers(self, k):
+        """(packets, lens, pk, sig, msg, out, stream) device addresses of set k."""
+        ptrs = [_vp() for _ in range(7)]
+        rc = self.lib.spl_verify_engine_set_buffers(self.h, k, *[ctypes.byref(p) for p in ptrs])
+        assert rc == 0
+        return tuple(p.value for p in ptrs)
 
     def run(self, n):
         return self.lib.spl_verify_engine_run(self.h, n)
+
+    def next(self):
+        """Index of the next buffer set, once its previous batch has finished."""
+        return self.lib.spl_verify_engine_next(self.h)
+
+    def run_set(self, k, n):
+        return self.lib.spl_verify_engine_run_set(self.h, k, n)
+
+    def sync_set(self, k):
+        return self.lib.spl_verify_engine_sync_set(self.h, k)
 
     def sync(self):
         return self.lib.spl_verify_engine_sync(self.h)

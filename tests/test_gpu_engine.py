@@ -47,6 +47,63 @@ def test_engine_replay_autotune(gpu_lib, oracle):
     assert np.array_equal(got, want)
 
 
+def _load_set(engine, k, b):
+    import hip_rt
+    bufs = engine.set_buffers(k)
+    for dptr, tab in zip(bufs[:5], (b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off)):
+        hip_rt.upload_to(dptr, tab)
+
+
+def test_engine_batches_in_flight(gpu_lib, oracle):
+    """An engine of 3 buffer sets with a different batch in each: rounds of
+    next() / run_set() keep up to 3 launches in flight, sets are refilled
+    with other batches between rounds, and every set's mask equals the
+    oracle's for the batch it holds; the same after autotuning with every set
+    in flight, in a persistent configuration (per-set work counters) and in
+    the tuned one."""
+    import hip_rt
+    import perf_libs
+    batches = [W.verify_validator(oracle.sign, oracle.create_keypair, seed=70 + k,
+                                  num_batches=6 + 3 * k, num_keys=64, flip_frac=0.1,
+                                  noncanon_frac=0.03, small_order_frac=0.03)
+               for k in range(4)]
+    wants = [oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off, threads=8)
+             for b in batches]
+    cap = max(b.n for b in batches)
+    eng = perf_libs.VerifyEngine(gpu_lib, cap, max(b.packets.size for b in batches), depth=3)
+    try:
+        held = [None] * 3
+
+        def rounds(count, refill):
+            for r in range(count):
+                k = eng.next()
+                assert 0 <= k < 3
+                # the set's previous batch is finished: check it, then maybe refill
+                if held[k] is not None:
+                    got = hip_rt.download_from(eng.set_buffers(k)[5], batches[held[k]].n)
+                    assert np.array_equal(got, wants[held[k]]), (k, held[k], eng.config())
+                if held[k] is None or refill:
+                    held[k] = (held[k] + 1 if held[k] is not None else k) % len(batches)
+                    _load_set(eng, k, batches[held[k]])
+                assert eng.run_set(k, batches[held[k]].n) == 0
+            assert eng.sync() == 0
+            for k in range(3):
+                got = hip_rt.download_from(eng.set_buffers(k)[5], batches[held[k]].n)
+                assert np.array_equal(got, wants[held[k]]), (k, held[k], eng.config())
+
+        rounds(9, refill=False)
+        rounds(12, refill=True)
+        names = eng.candidates()
+        eng.use(names.index("persistent W=4 scalars=half sort=1 waves_per_cu=4"))
+        rounds(9, refill=True)
+        # autotune with every set in flight (each set holds a batch of its own)
+        report = eng.autotune(min(batches[h].n for h in held))
+        assert len([l for l in report.splitlines() if l.strip()]) == len(names)
+        rounds(9, refill=True)
+    finally:
+        eng.close()
+
+
 def test_engine_bounds(gpu_lib, oracle):
     import perf_libs
     eng = perf_libs.VerifyEngine(gpu_lib, 64, 64 * 1232)
