@@ -201,6 +201,24 @@ int spl_verify_engine_config(spl_verify_engine *engine, char *buf, size_t len);
 int spl_verify_engine_candidates(spl_verify_engine *engine, char *buf, size_t len);
 int spl_verify_engine_use(spl_verify_engine *engine, int index);
 
+/* Tuning records (no counterpart in the reference).  The autotuner of a
+ * one-set engine records, per device and batch-size class (bit length of
+ * n), the fastest configuration and the fastest one-shot configuration; the
+ * ABI and device entry points use the latter.  spl_verify_tune_save writes
+ * every record to a text file (one line per record: device id, architecture,
+ * CU count, class, both configurations) and returns how many it wrote, < 0
+ * on a file error.  spl_verify_tune_load takes the records of such a file,
+ * so a restarted process need not tune again; records made on a device of
+ * another architecture or CU count than the same device id here are
+ * skipped.  It returns the number taken, or < 0 (nothing taken) when the
+ * file cannot be read or holds a malformed line or an unknown
+ * configuration.  spl_verify_tune_query writes the name of the one-shot
+ * configuration used for n signatures on `device` and returns 1 when it
+ * comes from a record, 0 for the untuned rule. */
+int spl_verify_tune_save(const char *path);
+int spl_verify_tune_load(const char *path);
+int spl_verify_tune_query(int device, uint32_t n, char *buf, size_t len);
+
 /* ------------------------------------------------------------------------
  * Extension: serving loop (no counterpart in the reference, whose callers
  * make one synchronous ed25519_verify_many / poh_verify_many call per batch).
@@ -230,8 +248,10 @@ int spl_verify_engine_use(spl_verify_engine *engine, int index);
  * written 0, PoH hashes are left unchanged).  On 0 and 3 the ticket is
  * retired and times_ns (if not NULL) receives submit, admit and completion
  * times from CLOCK_MONOTONIC.  spl_server_stats writes a JSON object with
- * per-kind request counts and p50/p99/max latency (submit to completion) and
- * optionally resets them.  spl_server_destroy finishes every submitted
+ * per-kind request counts and p50/p99/max latency (submit to completion),
+ * and the device operation the loop thread is inside at that moment with
+ * how long it has been there ("loop_in", "loop_in_ms"), and optionally
+ * resets the counts.  spl_server_destroy finishes every submitted
  * request first. */
 typedef struct spl_server spl_server;
 spl_server *spl_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch_signatures,

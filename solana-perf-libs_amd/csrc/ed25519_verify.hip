@@ -6,6 +6,8 @@
 // gpu_ctx.cu:75-144 (per-elem H2D ingest).
 #include <hip/hip_runtime.h>
 
+#include <stdio.h>
+
 #include <string>
 #include <vector>
 
@@ -521,6 +523,144 @@ int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
     }
     if (report && report_len) snprintf(report, report_len, "%s", rep.c_str());
     return 0;
+}
+
+// ------------------------------------------------------------ tuning records
+
+namespace {
+
+// "<arch> <cus>" of HIP device dev, or "" when it cannot be queried.
+std::string device_identity(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return "";
+    }
+    std::string arch = prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);   // drop feature flags
+    return arch + " " + std::to_string(prop.multiProcessorCount);
+}
+
+// Index of the candidate configuration called `name`, or -1.
+int candidate_index(const std::string &name) {
+    VerifyConfig cand[64];
+    const int nc = verify_config_candidates(cand, 64);
+    for (int i = 0; i < nc; i++) {
+        char buf[128];
+        config_name(cand[i], buf, sizeof buf);
+        if (name == buf) return i;
+    }
+    return -1;
+}
+
+int hip_device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+}  // namespace
+
+// One line per recorded (device, size class):
+//   device <id> <arch> <cus> class <c> best <configuration> | plain <configuration>
+// where a configuration is spelled as spl_verify_engine_config prints it.
+int spl_verify_tune_save(const char *path) {
+    if (!path) return -2;
+    FILE *f = fopen(path, "w");
+    if (!f) return -3;
+    fprintf(f, "# libcuda-crypt verify tuning records, one per device and batch-size class\n");
+    int count = 0;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (int d = 0; d < kMaxDevices; d++) {
+        std::string id;
+        for (int c = 0; c < 33; c++) {
+            const TuneEntry &t = g_tune[d][c];
+            if (!t.valid) continue;
+            if (id.empty()) id = device_identity(d);
+            char best[128], plain[128];
+            config_name(t.best, best, sizeof best);
+            config_name(t.best_plain, plain, sizeof plain);
+            fprintf(f, "device %d %s class %d best %s | plain %s\n", d,
+                    id.empty() ? "unknown 0" : id.c_str(), c, best, plain);
+            count++;
+        }
+    }
+    return fclose(f) == 0 ? count : -3;
+}
+
+// Returns the number of records taken, or < 0 when the file cannot be read
+// or holds a malformed line (then nothing is taken).  A record for a device
+// this process sees with another architecture or CU count is skipped.
+int spl_verify_tune_load(const char *path) {
+    if (!path) return -2;
+    FILE *f = fopen(path, "r");
+    if (!f) return -3;
+    struct Rec {
+        int dev, cls;
+        VerifyConfig best, plain;
+    };
+    std::vector<Rec> recs;
+    VerifyConfig cand[64];
+    verify_config_candidates(cand, 64);
+    const int ndev = hip_device_count();
+    char line[512];
+    bool bad = false;
+    while (fgets(line, sizeof line, f)) {
+        std::string ln(line);
+        while (!ln.empty() && (ln.back() == '\n' || ln.back() == '\r')) ln.pop_back();
+        if (ln.empty() || ln[0] == '#') continue;
+        int dev = -1, cus = 0, cls = -1, used = 0;
+        char arch[64];
+        if (sscanf(ln.c_str(), "device %d %63s %d class %d best %n", &dev, arch, &cus, &cls,
+                   &used) != 4 ||
+            used == 0 || dev < 0 || dev >= kMaxDevices || cls < 0 || cls > 32) {
+            bad = true;
+            break;
+        }
+        const std::string rest = ln.substr((size_t)used);
+        const size_t bar = rest.find(" | plain ");
+        if (bar == std::string::npos) {
+            bad = true;
+            break;
+        }
+        const int ib = candidate_index(rest.substr(0, bar));
+        const int ip = candidate_index(rest.substr(bar + 9));
+        if (ib < 0 || ip < 0 || cand[ip].persistent) {
+            bad = true;
+            break;
+        }
+        if (dev < ndev && device_identity(dev) != std::string(arch) + " " + std::to_string(cus))
+            continue;                                 // recorded on another kind of GPU
+        recs.push_back(Rec{dev, cls, cand[ib], cand[ip]});
+    }
+    fclose(f);
+    if (bad) return -4;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (const Rec &r : recs) {
+        TuneEntry &t = g_tune[r.dev][r.cls];
+        t.valid = true;
+        t.best = r.best;
+        t.best_plain = r.plain;
+    }
+    return (int)recs.size();
+}
+
+// The one-shot configuration the ABI and device entry points use for n
+// signatures on HIP device `device`; returns 1 when it comes from a tuning
+// record, 0 when it is the untuned rule, < 0 on bad arguments.
+int spl_verify_tune_query(int device, uint32_t n, char *buf, size_t len) {
+    if (device < 0 || device >= kMaxDevices || !buf || !len) return -2;
+    bool tuned;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        tuned = g_tune[device][size_class(n)].valid;
+    }
+    config_name(tuned_plain_config(device, n), buf, len);
+    return tuned ? 1 : 0;
 }
 
 }  // extern "C"

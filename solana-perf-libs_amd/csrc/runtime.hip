@@ -70,6 +70,10 @@ hipError_t Buf::reserve(size_t bytes) {
     hipError_t e = host ? hipHostMalloc(&ptr, want, hipHostMallocCoherent)
                         : hipMalloc(&ptr, want);
     if (e != hipSuccess) {
+        // The failure is returned here; it must not stay behind as the
+        // thread's last error, which the launch helpers read to detect a
+        // failed launch (a later call on this thread would fail with it).
+        (void)hipGetLastError();
         ptr = nullptr;
         cap = 0;
         return e;

@@ -34,7 +34,9 @@
 // the resources go back to the pool for the next server on the device.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "server_core.h"
@@ -161,6 +163,20 @@ struct HipDev {
     const ed25519_tables *T = nullptr;
     uint8_t *d_ring = nullptr, *h_ring = nullptr;
     std::vector<SlotRes *> res;
+    // The device operation the loop thread is inside, if any, and since when
+    // (read by spl_server_stats from any thread): a request that never
+    // completes then names the call the loop is waiting in.
+    std::atomic<const char *> in_call{nullptr};
+    std::atomic<uint64_t> in_call_since{0};
+
+    struct InCall {
+        HipDev &d;
+        InCall(HipDev &dv, const char *what) : d(dv) {
+            d.in_call_since.store(serve::now_ns(), std::memory_order_relaxed);
+            d.in_call.store(what, std::memory_order_release);
+        }
+        ~InCall() { d.in_call.store(nullptr, std::memory_order_release); }
+    };
 
     static bool ok(hipError_t e, const char *what) {
         if (e == hipSuccess) return true;
@@ -168,7 +184,10 @@ struct HipDev {
         return false;
     }
 
-    void thread_init() { (void)hipSetDevice(dev); }
+    void thread_init() {
+        InCall c(*this, "hipSetDevice");
+        (void)hipSetDevice(dev);
+    }
 
     // One PoH slice of batch b: hashes [b.poh_done, + kPohSliceHashes) of
     // every chain, then the egress copy if that was the last slice.
@@ -188,6 +207,7 @@ struct HipDev {
     }
 
     bool launch(int s, Batch &b) {
+        InCall c(*this, b.kind == kVerify ? "verify launch" : "PoH launch");
         SlotRes &r = *res[s];
         uint8_t *d = d_ring + b.off, *h = h_ring + b.off;
         if (b.kind == kVerify) {
@@ -215,7 +235,11 @@ struct HipDev {
     }
 
     Poll poll(int s) {
-        const hipError_t q = hipEventQuery(res[s]->done);
+        hipError_t q;
+        {
+            InCall c(*this, "hipEventQuery");
+            q = hipEventQuery(res[s]->done);
+        }
         if (q == hipErrorNotReady) return Poll::kBusy;
         if (q == hipSuccess) return Poll::kDone;
         fprintf(stderr, "spl_server: batch failed: %s\n", hipGetErrorString(q));
@@ -224,10 +248,25 @@ struct HipDev {
 
     int next(int s, Batch &b) {
         if (b.kind == kVerify || b.poh_done >= b.poh_max) return 0;
+        InCall c(*this, "PoH slice launch");
         return poh_slice(*res[s], b) ? 1 : -1;
     }
 
-    void drain(int s) { (void)hipStreamSynchronize(res[s]->stream); }
+    void drain(int s) {
+        InCall c(*this, "hipStreamSynchronize");
+        (void)hipStreamSynchronize(res[s]->stream);
+    }
+
+    // `"loop_in": "<operation>" | null, "loop_in_ms": <ms>`
+    std::string state_json() const {
+        const char *what = in_call.load(std::memory_order_acquire);
+        char buf[160];
+        if (!what) return "\"loop_in\": null, \"loop_in_ms\": 0";
+        const double ms =
+            (double)(serve::now_ns() - in_call_since.load(std::memory_order_relaxed)) / 1e6;
+        snprintf(buf, sizeof buf, "\"loop_in\": \"%s\", \"loop_in_ms\": %.3f", what, ms);
+        return buf;
+    }
 };
 
 }  // namespace
@@ -350,7 +389,14 @@ int spl_server_wait(spl_server *sv, int64_t ticket, int timeout_ms, uint64_t *ti
 
 int spl_server_stats(spl_server *sv, char *buf, size_t len, int reset) {
     if (!sv || !buf || !len) return 2;
-    return sv->core->stats(buf, len, reset != 0);
+    std::string s(65536, '\0');
+    if (sv->core->stats(&s[0], s.size(), reset != 0) != 0) return 2;
+    s.resize(strlen(s.c_str()));
+    // the core's object, with the loop thread's device operation added
+    s.pop_back();
+    s += ", " + sv->dev.state_json() + "}";
+    snprintf(buf, len, "%s", s.c_str());
+    return s.size() < len ? 0 : 2;
 }
 
 }  // extern "C"

@@ -36,7 +36,8 @@ EXPORTS = (
     "spl_verify_engine_create", "spl_verify_engine_destroy", "spl_verify_engine_buffers",
     "spl_verify_engine_create_pipelined", "spl_verify_engine_depth",
     "spl_verify_engine_set_buffers", "spl_verify_engine_next", "spl_verify_engine_run_set",
-    "spl_verify_engine_sync_set",
+    "spl_verify_engine_sync_set", "spl_verify_tune_save", "spl_verify_tune_load",
+    "spl_verify_tune_query",
     "spl_verify_engine_run", "spl_verify_engine_sync", "spl_verify_engine_autotune",
     "spl_verify_engine_config", "spl_verify_engine_candidates", "spl_verify_engine_use",
     "spl_server_create", "spl_server_destroy",
@@ -115,6 +116,13 @@ class PerfLibs:
         lib.spl_verify_engine_run_set.restype = ctypes.c_int
         lib.spl_verify_engine_sync_set.argtypes = [_vp, ctypes.c_uint32]
         lib.spl_verify_engine_sync_set.restype = ctypes.c_int
+        lib.spl_verify_tune_save.argtypes = [ctypes.c_char_p]
+        lib.spl_verify_tune_save.restype = ctypes.c_int
+        lib.spl_verify_tune_load.argtypes = [ctypes.c_char_p]
+        lib.spl_verify_tune_load.restype = ctypes.c_int
+        lib.spl_verify_tune_query.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_char_p,
+                                              ctypes.c_size_t]
+        lib.spl_verify_tune_query.restype = ctypes.c_int
         lib.spl_verify_engine_buffers.argtypes = [_vp] + [ctypes.POINTER(_vp)] * 7
         lib.spl_verify_engine_buffers.restype = ctypes.c_int
         lib.spl_verify_engine_run.argtypes = [_vp, ctypes.c_uint32]
@@ -262,6 +270,22 @@ class PerfLibs:
                            stream=0):
         return self.lib.ed25519_verify_many_device(d_packets, packets_size, d_lens, d_pk, d_sig,
                                                    d_msg, n, d_out, stream)
+
+    def tune_save(self, path):
+        """Writes the autotuner's records to `path`; returns how many."""
+        return self.lib.spl_verify_tune_save(os.fsencode(path))
+
+    def tune_load(self, path):
+        """Takes the records in `path`; returns how many (< 0: none, bad file)."""
+        return self.lib.spl_verify_tune_load(os.fsencode(path))
+
+    def tune_query(self, device, n):
+        """(from_record, configuration name) used for n signatures on `device`."""
+        buf = ctypes.create_string_buffer(256)
+        rc = self.lib.spl_verify_tune_query(device, n, buf, len(buf))
+        if rc < 0:
+            raise ValueError(f"spl_verify_tune_query({device}, {n}) = {rc}")
+        return bool(rc), buf.value.decode()
 
     def verify_many_device_noalloc(self, d_packets, packets_size, d_lens, d_pk, d_sig, d_msg, n,
                                    d_out, stream=0):

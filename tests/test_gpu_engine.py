@@ -22,7 +22,7 @@ def _mask(engine, n):
     return hip_rt.download_from(engine.d_out, n)
 
 
-def test_engine_replay_autotune(gpu_lib, oracle):
+def test_engine_replay_autotune(gpu_lib, oracle, tmp_path):
     import perf_libs
     b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=21, num_batches=24,
                            num_keys=128, flip_frac=0.1, noncanon_frac=0.03,
@@ -45,6 +45,16 @@ def test_engine_replay_autotune(gpu_lib, oracle):
     # the ABI call now uses the tuned one-shot configuration of this size class
     got = gpu_lib.verify_many(b.batches, b.message_size, b.lens, b.pk_off, b.sig_off, b.msg_off)
     assert np.array_equal(got, want)
+    # the record survives a save / load round trip, and the device's own
+    # architecture and CU count are written with it
+    tuned, plain = gpu_lib.tune_query(0, b.n)
+    assert tuned and "persistent" not in plain
+    path = tmp_path / "tune.txt"
+    assert gpu_lib.tune_save(str(path)) >= 1
+    line = [l for l in path.read_text().splitlines() if " class %d " % b.n.bit_length() in l]
+    assert len(line) == 1 and line[0].startswith("device 0 gfx950 ") and line[0].endswith(plain)
+    assert gpu_lib.tune_load(str(path)) >= 1
+    assert gpu_lib.tune_query(0, b.n) == (True, plain)
 
 
 def _load_set(engine, k, b):

@@ -124,6 +124,52 @@ def test_empty_and_bad_offsets(gpu_lib, oracle):
     assert got.tolist() == [1, 1, 1, 0, 1, 0, 1, 1]
 
 
+def test_allocation_failure_rejects_and_recovers(gpu_lib, oracle):
+    """Calls whose buffers cannot be allocated on the device (a 1 TiB packet
+    area or hash array, more than the GPU holds) fail safely instead of
+    aborting as the reference does on a runtime error (gpu_common.h:15-24):
+    verify rejects every signature, sign writes zero signatures, PoH returns 1
+    with the hashes untouched; nothing of the caller's buffers is read.  The
+    same thread's next calls, on every queue of the pool, give exact results."""
+    import ctypes
+    import perf_libs
+    L = gpu_lib.lib
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    empty = np.zeros(64, np.uint8)
+    elems = (perf_libs.GpuElems * 1)()
+    elems[0].elems = empty.ctypes.data_as(u8p)
+    elems[0].num = 0                                  # no packet bytes to read
+    n = 4
+    tabs = [np.zeros(n, np.uint32) for _ in range(4)]
+    tp = [t.ctypes.data_as(u32p) for t in tabs]
+    for _ in range(8):                                # every queue of the pool
+        out = np.full(n, 7, np.uint8)
+        L.ed25519_verify_many(elems, 1, 1 << 20, 1 << 20, n, *tp, out.ctypes.data_as(u8p), 1)
+        assert out.tolist() == [0] * n
+        sigs = np.full(64 * n, 7, np.uint8)
+        L.ed25519_sign_many(elems, 1, 1 << 20, 1 << 20, n, *tp, sigs.ctypes.data_as(u8p), 1)
+        assert not sigs.any()
+        h = np.arange(64, dtype=np.uint8)
+        counts = np.ones(2, np.uint64)
+        assert L.poh_verify_many(h.ctypes.data_as(u8p),
+                                 counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                 1 << 35, 1) == 1
+        assert np.array_equal(h, np.arange(64, dtype=np.uint8))
+    b = W.verify_plain(96, oracle.sign, oracle.create_keypair, seed=5, corrupt_frac=0.2)
+    want = _oracle_mask(oracle, b)
+    h0, counts = W.poh_chains(40, 17, seed=5)
+    want_h = h0.copy()
+    oracle.poh_verify_many(want_h, counts)
+    for _ in range(8):
+        got = gpu_lib.verify_many(b.batches, b.message_size, b.lens, b.pk_off, b.sig_off,
+                                  b.msg_off)
+        assert np.array_equal(got, want)
+        h = h0.copy()
+        assert gpu_lib.poh_verify_many(h, counts) == 0
+        assert np.array_equal(h, want_h)
+
+
 def test_device_resident_form(gpu_lib, oracle):
     import hip_rt
     b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=11, num_batches=6,

@@ -136,6 +136,59 @@ print('ok')
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
 
 
+def test_tune_records_file(tmp_path):
+    """The autotuner's records in a file: load takes well-formed records
+    (the ABI then uses the recorded one-shot configuration for that
+    batch-size class), save writes back the same records, and a malformed
+    line or an unknown configuration makes load take nothing.  Without a
+    device, so no record can be skipped as made on another GPU."""
+    good = tmp_path / "tune.txt"
+    good.write_text(
+        "# comment\n"
+        "device 0 gfx950 256 class 16 best persistent W=4 scalars=half sort=1 waves_per_cu=4 "
+        "| plain one-shot W=4 scalars=half waves_per_eu=1\n"
+        "device 0 gfx950 256 class 10 best one-shot W=5 scalars=half lanes=2 waves_per_eu=2 "
+        "| plain one-shot W=5 scalars=half lanes=2 waves_per_eu=2\n")
+    bad_line = tmp_path / "bad.txt"
+    bad_line.write_text("device 0 gfx950 256 class 12 best\n")
+    bad_name = tmp_path / "badname.txt"
+    bad_name.write_text("device 0 gfx950 256 class 12 best one-shot W=7 scalars=half "
+                        "waves_per_eu=1 | plain one-shot W=4 scalars=half waves_per_eu=1\n")
+    persistent_plain = tmp_path / "pp.txt"
+    persistent_plain.write_text(
+        "device 0 gfx950 256 class 12 best one-shot W=4 scalars=half waves_per_eu=1 "
+        "| plain persistent W=4 scalars=half sort=1 waves_per_cu=4\n")
+    saved = tmp_path / "saved.txt"
+    code = rf"""
+import sys
+sys.path.insert(0, 'solana-perf-libs_amd')
+import perf_libs
+L = perf_libs.load()
+before = L.tune_query(0, 3000)
+assert before[0] is False, before
+for p in ({str(bad_line)!r}, {str(bad_name)!r}, {str(persistent_plain)!r}):
+    assert L.tune_load(p) == -4, p
+assert L.tune_query(0, 3000) == before
+assert L.tune_load({str(tmp_path / 'missing.txt')!r}) == -3
+assert L.tune_load({str(good)!r}) == 2
+assert L.tune_query(0, 40000) == (True, 'one-shot W=4 scalars=half waves_per_eu=1')
+assert L.tune_query(0, 33000) == (True, 'one-shot W=4 scalars=half waves_per_eu=1')
+assert L.tune_query(0, 600) == (True, 'one-shot W=5 scalars=half lanes=2 waves_per_eu=2')
+assert L.tune_query(0, 3000) == before
+assert L.tune_query(1, 40000)[0] is False
+assert L.tune_save({str(saved)!r}) == 2
+print('ok')
+"""
+    r = _run_without_gpu(code)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
+    lines = [l for l in saved.read_text().splitlines() if not l.startswith("#")]
+    assert lines == [
+        "device 0 unknown 0 class 10 best one-shot W=5 scalars=half lanes=2 waves_per_eu=2 "
+        "| plain one-shot W=5 scalars=half lanes=2 waves_per_eu=2",
+        "device 0 unknown 0 class 16 best persistent W=4 scalars=half sort=1 waves_per_cu=4 "
+        "| plain one-shot W=4 scalars=half waves_per_eu=1"], lines
+
+
 def test_check_helpers_full_mix(lib, oracle):
     """Both host check helpers over 100k random 32-byte encodings, 1 % of them
     replaced by small-order encodings (canonical and non-canonical) and 1 %
