@@ -264,11 +264,12 @@ def abi_callers_leg(lib, b, mask, calls=8, thread_counts=(1, 2, 4)):
     return out
 
 
-def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None):
+def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None, slots=(0, 0)):
     """Served mixed load: Poisson verify + PoH requests through spl_server at
     stepped offered loads, beside back-to-back plain ed25519_verify_many calls
     on the same requests.  progress() is called after each step; `partial`
-    (a dict) receives the steps measured so far as they complete."""
+    (a dict) receives the steps measured so far as they complete.  slots:
+    (verify, PoH) streams of the server (0: the library's defaults)."""
     partial = {} if partial is None else partial
     import loadgen
     import perf_libs
@@ -280,7 +281,8 @@ def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None):
     rng = np.random.default_rng(12)
     steps = []
     log("bench: served leg: creating the server")
-    server = perf_libs.Server(lib, hbm_budget_bytes=1 << 30)
+    server = perf_libs.Server(lib, hbm_budget_bytes=1 << 30, verify_streams=slots[0],
+                              poh_streams=slots[1])
     progress()
     error = None
     try:
@@ -494,6 +496,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the PoH and plain-ABI legs")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-load leg")
+    ap.add_argument("--served-slots", default="0,0",
+                    help="verify,PoH streams of the served leg's server (0: library defaults)")
+    ap.add_argument("--served-only", action="store_true",
+                    help="run only the served-load leg (with its request pools)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before timing "
                          "(a known-size dispatch that calibrates HBM byte counters)")
@@ -513,6 +519,19 @@ def main():
 
     log("bench: generating validator-shaped batch")
     b = W.verify_validator(lib.sign, lib.create_keypair, seed=3)
+    slots = tuple(int(x) for x in args.served_slots.split(","))
+    if args.served_only:
+        mask = lib.verify_many(b.batches, b.message_size, b.lens, b.pk_off, b.sig_off, b.msg_off)
+        res = {"served_slots": list(slots)}
+        res["served"] = served_leg(lib, b, mask, slots=slots)
+        global DETAIL_PATH
+        DETAIL_PATH = DETAIL_PATH.replace(".json", "_served_%d_%d.json" % slots)
+        os.makedirs(os.path.dirname(DETAIL_PATH), exist_ok=True)
+        with open(DETAIL_PATH, "w") as f:
+            json.dump({**res, "log": _NOTES}, f, indent=1)
+        best = res["served"].get("best_served_at_p99_le_2ms")
+        print(json.dumps({"served_slots": list(slots), "best_at_p99_le_2ms": best}), flush=True)
+        return
 
     if args.calib_copy_mb:
         nbytes = args.calib_copy_mb << 20
@@ -721,7 +740,7 @@ def main():
         dog = Watchdog(emit, result, "served", limit_s=150.0)
         result["served"] = {"partial": True}
         result["served"] = served_leg(lib, b, mask, progress=dog.kick,
-                                      partial=result["served"])
+                                      partial=result["served"], slots=slots)
         dog.close()
     emit(result)
 
