@@ -125,7 +125,10 @@ hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t
 // of the next.  On return h[0, total) is written (zeros after the elems'
 // bytes if they hold fewer), and the first *sent bytes (a multiple of 16) are
 // enqueued for d; the caller fills what follows in h and copies [*sent, end)
-// itself.
+// itself.  An elem whose bytes are already in pinned host memory (allocated
+// pinned, or registered with cuda_host_register) is copied to d straight from
+// the caller's pages instead, when its offset in the concatenation, its size
+// and its address are multiples of 16.
 constexpr uint64_t kStageChunk = 4ull << 20;
 hipError_t stage_elems(uint8_t *h, uint8_t *d, const gpu_Elems *elems, uint32_t num_elems,
                        uint32_t stride, uint64_t total, uint64_t *sent, hipStream_t s);

@@ -42,6 +42,8 @@ _hip.hipGetErrorString.restype = ctypes.c_char_p
 _hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
 _hip.hipSetDevice.argtypes = [ctypes.c_int]
 _hip.hipDeviceGetAttribute.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
+_hip.hipHostMalloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t, ctypes.c_uint]
+_hip.hipHostFree.argtypes = [_vp]
 
 H2D, D2H, D2D = 1, 2, 3
 
@@ -113,6 +115,35 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class _PinnedBlock:
+    """hipHostMalloc'd bytes, freed when the last array viewing them goes."""
+
+    def __init__(self, nbytes):
+        p = _vp()
+        check(_hip.hipHostMalloc(ctypes.byref(p), max(1, nbytes), 0), "hipHostMalloc")
+        self.ptr = p.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                _hip.hipHostFree(self.ptr)
+                self.ptr = 0
+        except Exception:
+            pass
+
+
+def pinned_copy(a):
+    """A uint8 copy of `a` in pinned host memory (hipHostMalloc), as a numpy
+    array that keeps the allocation alive."""
+    a = np.ascontiguousarray(a, dtype=np.uint8).reshape(-1)
+    block = _PinnedBlock(a.nbytes)
+    buf = (ctypes.c_uint8 * max(1, a.nbytes)).from_address(block.ptr)
+    buf._block = block
+    out = np.frombuffer(buf, dtype=np.uint8, count=a.nbytes)
+    out[:] = a
+    return out
 
 
 class Stream:

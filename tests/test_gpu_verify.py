@@ -170,6 +170,45 @@ def test_allocation_failure_rejects_and_recovers(gpu_lib, oracle):
         assert np.array_equal(h, want_h)
 
 
+def test_pinned_packet_batches(gpu_lib, oracle):
+    """Packet batches the caller holds in pinned memory are read by the copy
+    kernel straight from the caller's pages; others go through the staging
+    buffer.  A call mixing pinned batches, pageable ones and a pinned batch at
+    an address that is not 16-byte aligned (staged) gives the oracle's mask,
+    for verify, sign and the wire-format packet entry point alike."""
+    import hip_rt
+    b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=31, num_batches=12,
+                           num_keys=64, flip_frac=0.1)
+    want = _oracle_mask(oracle, b)
+    mixed = []
+    for k, x in enumerate(b.batches):
+        if k % 3 == 0:
+            mixed.append(hip_rt.pinned_copy(x))               # pinned, aligned
+        elif k % 3 == 1:
+            mixed.append(x.copy())                            # pageable
+        else:
+            pad = hip_rt.pinned_copy(np.concatenate([np.zeros(8, np.uint8), x]))
+            mixed.append(pad[8:])                             # pinned, misaligned: staged
+    assert any(m.ctypes.data % 16 for m in mixed)
+    for batches in (mixed, [hip_rt.pinned_copy(x) for x in b.batches]):
+        for _ in range(3):
+            got = gpu_lib.verify_many(batches, b.message_size, b.lens, b.pk_off, b.sig_off,
+                                      b.msg_off)
+            assert np.array_equal(got, want)
+    s = W.sign_batch(300, oracle.create_keypair, seed=33)
+    want_s = oracle.sign_many(s.packets, s.lens, s.pk_off, s.sig_off, s.msg_off)
+    got_s = gpu_lib.sign_many([hip_rt.pinned_copy(x) for x in s.batches], s.message_size,
+                              s.lens, s.pk_off, s.sig_off, s.msg_off)
+    assert np.array_equal(got_s, want_s)
+    wp = W.wire_packets(oracle.sign, oracle.create_keypair, seed=34, num_batches=6,
+                        malformed_frac=0.05)
+    rc, ok_ref, _ = gpu_lib.verify_packets(wp.batches, wp.message_size, wp.meta["sizes"])
+    assert rc == 0
+    rc, ok_pinned, _ = gpu_lib.verify_packets([hip_rt.pinned_copy(x) for x in wp.batches],
+                                              wp.message_size, wp.meta["sizes"])
+    assert rc == 0 and np.array_equal(ok_pinned, ok_ref) and 0 < ok_ref.sum() < ok_ref.size
+
+
 def test_device_resident_form(gpu_lib, oracle):
     import hip_rt
     b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=11, num_batches=6,
