@@ -283,12 +283,15 @@ def abi_callers_leg(lib, b, mask, calls=8, thread_counts=(1, 2, 4)):
     return out
 
 
-def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None, slots=(0, 0)):
+def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None, slots=(0, 0),
+               pinned=False):
     """Served mixed load: Poisson verify + PoH requests through spl_server at
     stepped offered loads, beside back-to-back plain ed25519_verify_many calls
     on the same requests.  progress() is called after each step; `partial`
     (a dict) receives the steps measured so far as they complete.  slots:
-    (verify, PoH) streams of the server (0: the library's defaults)."""
+    (verify, PoH) streams of the server (0: the library's defaults).  pinned:
+    the requests' packets live in pinned host memory, so the server and the
+    plain calls read them in place."""
     partial = {} if partial is None else partial
     import loadgen
     import perf_libs
@@ -296,10 +299,11 @@ def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None, s
     poh_in, counts = W.poh_chains(512, 12500, seed=8)
     poh_out = poh_in.copy()
     assert lib.poh_verify_many(poh_out, counts) == 0
-    pools = loadgen.RequestPools(lib, b, mask, poh_in, counts, poh_out)
+    pools = loadgen.RequestPools(lib, b, mask, poh_in, counts, poh_out, pinned=pinned)
     rng = np.random.default_rng(12)
     steps = []
-    log("bench: served leg: creating the server")
+    log("bench: served leg (%s packets): creating the server"
+        % ("pinned" if pinned else "pageable"))
     server = perf_libs.Server(lib, hbm_budget_bytes=1 << 30, verify_streams=slots[0],
                               poh_streams=slots[1])
     progress()
@@ -338,6 +342,7 @@ def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None, s
     return {"workload": "Poisson verify requests (log-uniform 1-2048 packets of the validator "
                         "batch) + PoH requests (50/s, log-uniform 1-512 chains x 12,500 "
                         "hashes), %.1f s per offered-load step, 1 GiB device ring" % seconds,
+            "packets_in": "pinned host memory" if pinned else "pageable host memory",
             "steps": steps, "back_to_back_plain": b2b,
             "best_served_at_p99_le_2ms": None if best is None else {
                 "offered_sigs_per_s": best["offered_sigs_per_s"],
@@ -519,6 +524,8 @@ def main():
                     help="verify,PoH streams of the served leg's server (0: library defaults)")
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served-load leg (with its request pools)")
+    ap.add_argument("--served-pinned", action="store_true",
+                    help="with --served-only: the requests' packets in pinned host memory")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before timing "
                          "(a known-size dispatch that calibrates HBM byte counters)")
@@ -541,15 +548,19 @@ def main():
     slots = tuple(int(x) for x in args.served_slots.split(","))
     if args.served_only:
         mask = lib.verify_many(b.batches, b.message_size, b.lens, b.pk_off, b.sig_off, b.msg_off)
-        res = {"served_slots": list(slots)}
-        res["served"] = served_leg(lib, b, mask, slots=slots)
+        res = {"served_slots": list(slots), "served_pinned": args.served_pinned}
+        res["served"] = served_leg(lib, b, mask, slots=slots, pinned=args.served_pinned)
         global DETAIL_PATH
-        DETAIL_PATH = DETAIL_PATH.replace(".json", "_served_%d_%d.json" % slots)
+        DETAIL_PATH = DETAIL_PATH.replace(".json", "_served_%d_%d%s.json" % (
+            slots + ("_pinned" if args.served_pinned else "",)))
         os.makedirs(os.path.dirname(DETAIL_PATH), exist_ok=True)
         with open(DETAIL_PATH, "w") as f:
             json.dump({**res, "log": _NOTES}, f, indent=1)
         best = res["served"].get("best_served_at_p99_le_2ms")
-        print(json.dumps({"served_slots": list(slots), "best_at_p99_le_2ms": best}), flush=True)
+        print(json.dumps({"served_slots": list(slots), "served_pinned": args.served_pinned,
+                          "best_at_p99_le_2ms": best,
+                          "back_to_back_plain": res["served"].get("back_to_back_plain")}),
+              flush=True)
         return
 
     if args.calib_copy_mb:

@@ -237,10 +237,13 @@ int spl_verify_tune_query(int device, uint32_t n, char *buf, size_t len);
  * spl_server_submit_verify takes one request in the ed25519_verify_many
  * layout, with offsets relative to `packets` (packets_size bytes); a signature
  * whose offsets fall outside its own packets is rejected (0).  `out` receives
- * num_signatures mask bytes.  spl_server_submit_poh hashes `hashes` in place
- * like poh_verify_many.  Every pointer passed must stay valid until
- * spl_server_wait reports the ticket complete.  Tickets < 0: -2 bad arguments
- * or a request larger than the ring or max_batch_signatures.
+ * num_signatures mask bytes.  Packets in pinned host memory (allocated
+ * pinned or registered, with a 16-byte-aligned address and size) are read by
+ * the device in place; others are copied by the loop.  spl_server_submit_poh
+ * hashes `hashes` in place like poh_verify_many.  Every buffer passed must
+ * stay valid and unchanged until spl_server_wait reports the ticket
+ * complete.  Tickets < 0: -2 bad arguments or a request larger than the ring
+ * or max_batch_signatures.
  *
  * spl_server_wait blocks up to timeout_ms (< 0: no limit) and returns 0 when
  * the results are written, 1 on time-out (the ticket stays valid), 2 for an
@@ -248,11 +251,11 @@ int spl_verify_tune_query(int device, uint32_t n, char *buf, size_t len);
  * written 0, PoH hashes are left unchanged).  On 0 and 3 the ticket is
  * retired and times_ns (if not NULL) receives submit, admit and completion
  * times from CLOCK_MONOTONIC.  spl_server_stats writes a JSON object with
- * per-kind request counts and p50/p99/max latency (submit to completion),
- * and the device operation the loop thread is inside at that moment with
- * how long it has been there ("loop_in", "loop_in_ms"), and optionally
- * resets the counts.  spl_server_destroy finishes every submitted
- * request first. */
+ * per-kind request counts (for verify, also how many were read in place) and
+ * p50/p99/max latency (submit to completion), and the device operation the
+ * loop thread is inside at that moment with how long it has been there
+ * ("loop_in", "loop_in_ms"), and optionally resets the counts.
+ * spl_server_destroy finishes every submitted request first. */
 typedef struct spl_server spl_server;
 spl_server *spl_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch_signatures,
                               uint32_t verify_streams, uint32_t poh_streams);

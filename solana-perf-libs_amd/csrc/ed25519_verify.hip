@@ -527,6 +527,8 @@ int spl_verify_engine_autotune(spl_verify_engine *eng, uint32_t n, char *report,
 
 // ------------------------------------------------------------ tuning records
 
+}  // extern "C"
+
 namespace {
 
 // "<arch> <cus>" of HIP device dev, or "" when it cannot be queried.
@@ -564,6 +566,8 @@ int hip_device_count() {
 }
 
 }  // namespace
+
+extern "C" {
 
 // One line per recorded (device, size class):
 //   device <id> <arch> <cus> class <c> best <configuration> | plain <configuration>

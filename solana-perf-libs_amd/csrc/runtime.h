@@ -117,6 +117,41 @@ void *stream_scratch(int hip_device, hipStream_t stream, size_t bytes);
 // rounded length).  Used instead of DMA copies for host<->device transfers.
 hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 
+// The device address of the host bytes [p, p + n) when all of them lie in
+// one allocation the GPU can read directly: pinned host memory, allocated as
+// such or registered (cuda_host_register).  nullptr for anything else.
+const uint8_t *pinned_device_range(const uint8_t *p, uint64_t n);
+
+// Up to kGatherMax copies of 16-byte-aligned ranges (device or pinned host
+// memory) into one device buffer, done by one kernel launch whose blocks each
+// copy one kGatherChunk16-word chunk of one range; the list travels as the
+// kernel's argument.  first[k] is range k's first block, first[count] the
+// grid size.
+constexpr uint32_t kGatherMax = 128;
+constexpr uint32_t kGatherChunk16 = 1024;    // 16 KB per block
+struct GatherList {
+    uint32_t count;
+    uint32_t first[kGatherMax + 1];
+    uint32_t n16[kGatherMax];
+    uint64_t dst_off[kGatherMax];
+    const uint4 *src[kGatherMax];
+    GatherList() : count(0) { first[0] = 0; }
+    bool full() const { return count == kGatherMax; }
+    // Appends [src, src + 16 n16) -> d + dst_off (0 < n16 < 2^32, list not
+    // full; dst_off and src 16-byte aligned).
+    void push(uint64_t off, const void *src_, uint64_t words) {
+        dst_off[count] = off;
+        src[count] = static_cast<const uint4 *>(src_);
+        n16[count] = (uint32_t)words;
+        first[count + 1] = first[count] + (uint32_t)((words + kGatherChunk16 - 1) / kGatherChunk16);
+        count++;
+    }
+};
+static_assert(sizeof(GatherList) <= 4096, "kernel argument size");
+// Enqueues the list's copies into d on s (nothing when it is empty) and
+// empties it.
+hipError_t launch_gather(uint8_t *d, GatherList &list, hipStream_t s);
+
 // Copies the packets of elems[0..num_elems) (elems[k].num packets of
 // `stride` bytes each, concatenated, cut at `total` bytes) into pinned host
 // memory h, and sends them on to device memory d with copy kernels on s as

@@ -33,14 +33,11 @@ hipError_t launch_copy16(void *dst, const void *src, uint64_t bytes, hipStream_t
     return hipGetLastError();
 }
 
-// The device address of the host bytes [p, p + n) when all of them lie in
-// one allocation the GPU can read directly: pinned host memory, allocated as
-// such or registered (cuda_host_register).  nullptr for anything else.
 // Measured on MI355X: for pinned memory the runtime reports the queried
 // address itself as the device address (also inside the allocation) and the
 // allocation's start and size; ordinary memory reports type "unregistered"
 // and fails the range queries.
-static const uint8_t *pinned_device_range(const uint8_t *p, uint64_t n) {
+const uint8_t *pinned_device_range(const uint8_t *p, uint64_t n) {
     if (n == 0) return nullptr;
     hipPointerAttribute_t a;
     if (hipPointerGetAttributes(&a, p) != hipSuccess) {
@@ -61,36 +58,34 @@ static const uint8_t *pinned_device_range(const uint8_t *p, uint64_t n) {
     return static_cast<const uint8_t *>(a.devicePointer);
 }
 
-// Copies of up to kGatherMax pinned host ranges into one device buffer, in
-// one launch: the descriptors travel as the kernel's argument (3 KB), and
-// kGatherBlocks blocks share each range.  A validator batch arrives as
-// ~128 separate packet batches; one launch per batch left the copies
-// waiting on each other's launch and PCIe round trips.
-constexpr int kGatherMax = 128, kGatherBlocks = 4;
-struct GatherList {
-    uint32_t count;
-    uint64_t dst_off[kGatherMax];
-    const uint4 *src[kGatherMax];
-    uint64_t n16[kGatherMax];
-};
-
+// One block per 16 KB chunk of a range, so a large range gets as many blocks
+// reading over PCIe at once as it has chunks, and a list of small ranges (a
+// validator batch arrives as ~128 packet batches) goes in one launch: one
+// launch per range left the copies waiting on each other's launch and PCIe
+// round trips.
 __global__ void __launch_bounds__(256) gather_copy16_kernel(uint4 *__restrict__ dst,
                                                             GatherList list) {
     raise_wave_priority();
-    const uint32_t k = blockIdx.x / kGatherBlocks;
-    if (k >= list.count) return;
-    uint4 *out = dst + list.dst_off[k] / 16;
-    const uint4 *in = list.src[k];
-    const uint64_t n16 = list.n16[k];
-    const uint64_t step = (uint64_t)kGatherBlocks * blockDim.x;
-    for (uint64_t i = (uint64_t)(blockIdx.x % kGatherBlocks) * blockDim.x + threadIdx.x; i < n16;
-         i += step)
-        out[i] = in[i];
+    // the range holding this block's chunk: the last k with first[k] <= block
+    // (every range has at least one block, so first[] rises strictly)
+    const uint32_t blk = blockIdx.x;
+    uint32_t lo = 0, hi = list.count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (list.first[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    const uint64_t c0 = (uint64_t)(blk - list.first[lo]) * kGatherChunk16;
+    const uint64_t n16 = list.n16[lo];
+    const uint64_t end = c0 + kGatherChunk16 < n16 ? c0 + kGatherChunk16 : n16;
+    uint4 *out = dst + list.dst_off[lo] / 16;
+    const uint4 *in = list.src[lo];
+    for (uint64_t i = c0 + threadIdx.x; i < end; i += blockDim.x) out[i] = in[i];
 }
 
-static hipError_t launch_gather(uint8_t *d, GatherList &list, hipStream_t s) {
+hipError_t launch_gather(uint8_t *d, GatherList &list, hipStream_t s) {
     if (list.count == 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_copy16_kernel, dim3(list.count * kGatherBlocks), dim3(256), 0, s,
+    hipLaunchKernelGGL(gather_copy16_kernel, dim3(list.first[list.count]), dim3(256), 0, s,
                        reinterpret_cast<uint4 *>(d), list);
     list.count = 0;
     return hipGetLastError();
@@ -100,7 +95,6 @@ hipError_t stage_elems(uint8_t *h, uint8_t *d, const gpu_Elems *elems, uint32_t 
                        uint32_t stride, uint64_t total, uint64_t *sent, hipStream_t s) {
     uint64_t cur = 0, done = 0;
     GatherList pinned;
-    pinned.count = 0;
     for (uint32_t k = 0; k < num_elems && cur < total; k++) {
         uint64_t b = (uint64_t)elems[k].num * stride;
         if (cur + b > total) b = total - cur;   // never overrun
@@ -110,7 +104,8 @@ hipError_t stage_elems(uint8_t *h, uint8_t *d, const gpu_Elems *elems, uint32_t 
         // through the staging buffer (when its place in the concatenation,
         // its size and its address allow 16-byte copies).
         const uint8_t *dsrc = nullptr;
-        if (b && cur % 16 == 0 && b % 16 == 0 && (uintptr_t)src % 16 == 0)
+        if (b && cur % 16 == 0 && b % 16 == 0 && (uintptr_t)src % 16 == 0 &&
+            b / 16 < (1ull << 32))
             dsrc = pinned_device_range(src, b);
         if ((uintptr_t)dsrc % 16) dsrc = nullptr;
         if (dsrc) {
@@ -118,12 +113,9 @@ hipError_t stage_elems(uint8_t *h, uint8_t *d, const gpu_Elems *elems, uint32_t 
             // this elem's region to the gathered copy
             hipError_t e = hipSuccess;
             if (cur > done) e = launch_copy16(d + done, h + done, cur - done, s);
-            if (e == hipSuccess && pinned.count == kGatherMax) e = launch_gather(d, pinned, s);
+            if (e == hipSuccess && pinned.full()) e = launch_gather(d, pinned, s);
             if (e != hipSuccess) return e;
-            pinned.dst_off[pinned.count] = cur;
-            pinned.src[pinned.count] = reinterpret_cast<const uint4 *>(dsrc);
-            pinned.n16[pinned.count] = b / 16;
-            pinned.count++;
+            pinned.push(cur, dsrc, b / 16);
             cur += b;
             done = cur;
             continue;

@@ -11,8 +11,10 @@
 // streams and the C ABI around it.
 //
 // Per batch, on the slot's stream: ingest copy kernel (pinned mirror -> ring
-// block), the verify kernel or one PoH slice, egress copy kernel (results ->
-// pinned mirror), event.  The loop thread polls the events.
+// block; a verify request whose packets are already in pinned memory is
+// gathered from the caller's pages instead), the verify kernel or one PoH
+// slice, egress copy kernel (results -> pinned mirror), event.  The loop
+// thread polls the events.
 //
 // Hardware queues.  Every slot stream is a plain, non-blocking, default-
 // priority stream, so all of them share the process's GPU_MAX_HW_QUEUES
@@ -30,7 +32,8 @@
 // verify scratch, hipHostMalloc of the pinned mirror, hipStreamCreateWithFlags
 // and hipEventCreateWithFlags per slot).  The loop thread makes hipSetDevice,
 // kernel launches, hipEventRecord and hipEventQuery, and hipStreamSynchronize
-// on a failed batch.  spl_server_destroy makes none: it joins the loop, and
+// on a failed batch.  spl_server_submit_verify asks hipPointerGetAttributes
+// (on the submitting thread) whether the packets lie in pinned memory.  spl_server_destroy makes none: it joins the loop, and
 // the resources go back to the pool for the next server on the device.
 #include <hip/hip_runtime.h>
 
@@ -206,6 +209,23 @@ struct HipDev {
         return ok(hipEventRecord(r.done, r.stream), "event record");
     }
 
+    // A staged verify batch's input into its ring block: one copy from the
+    // host copy, or, with requests read in place, gathered copies from the
+    // host copy and the callers' pinned pages (one launch per kGatherMax
+    // ranges).
+    hipError_t ingest_verify(SlotRes &r, const Batch &b) {
+        uint8_t *d = d_ring + b.off;
+        const uint8_t *h = h_ring + b.off;
+        if (!b.direct) return launch_copy16(d, h, b.in_bytes, r.stream);
+        GatherList g;
+        hipError_t e = hipSuccess;
+        serve::ingest_ranges(h, b, [&](const uint8_t *src, uint64_t off, uint64_t n) {
+            if (e == hipSuccess && g.full()) e = launch_gather(d, g, r.stream);
+            if (e == hipSuccess) g.push(off, src, n / 16);
+        });
+        return e == hipSuccess ? launch_gather(d, g, r.stream) : e;
+    }
+
     bool launch(int s, Batch &b) {
         InCall c(*this, b.kind == kVerify ? "verify launch" : "PoH launch");
         SlotRes &r = *res[s];
@@ -214,7 +234,7 @@ struct HipDev {
             serve::stage_verify(h_ring, b);
             const VerifyConfig cfg = tuned_plain_config(dev, b.n);
             if (!ok(ensure_scratch(r, verify_scratch_bytes(cfg, b.n, cus)), "scratch") ||
-                !ok(launch_copy16(d, h, b.in_bytes, r.stream), "ingest copy"))
+                !ok(ingest_verify(r, b), "ingest copy"))
                 return false;
             const uint64_t tab_off = align_up(b.packets_bytes, 256);
             const uint32_t *tab = reinterpret_cast<const uint32_t *>(d + tab_off);
@@ -366,6 +386,15 @@ int64_t spl_server_submit_verify(spl_server *sv, const uint8_t *packets, uint64_
     r->sig = signature_offsets;
     r->msg = message_start_offsets;
     r->out = out;
+    // packets already in pinned memory (how solana_perf holds its packet
+    // batches) are read by the device in place, not staged by the loop
+    if (num_signatures && packets_size % 16 == 0 && (uintptr_t)packets % 16 == 0) {
+        const uint8_t *dp = pinned_device_range(packets, packets_size);
+        if (dp && (uintptr_t)dp % 16 == 0) {
+            r->direct = true;
+            r->packets_dev = dp;
+        }
+    }
     return sv->core->submit(r);
 }
 

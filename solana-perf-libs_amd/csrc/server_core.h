@@ -20,13 +20,17 @@
 //   void drain(int slot);            wait until everything enqueued on slot
 //                                    has finished
 //
-// A verify batch is one step (ingest copy, verify kernel, egress copy).  A
-// PoH batch is one step per slice of at most `poh_slice` hashes per chain
-// (the first also copies the chains in, the last copies them out); the chain
-// state stays in the ring block between slices.  The loop enqueues the next
-// slice only when the previous one has finished, so a slot's queue never
-// holds more than one slice: any other work behind it waits at most one
-// slice, however long the chains are.
+// A verify batch is one step (ingest copy, verify kernel, egress copy).  The
+// loop stages each request's packets into the ring's host copy, except those
+// of a request whose packets already sit in pinned host memory: the ingest
+// copy reads those straight from the caller's pages (ingest_ranges), and the
+// loop writes only their rebased offset tables.  A PoH batch is one step per
+// slice of at most `poh_slice` hashes per chain (the first also copies the
+// chains in, the last copies them out); the chain state stays in the ring
+// block between slices.  The loop enqueues the next slice only when the
+// previous one has finished, so a slot's queue never holds more than one
+// slice: any other work behind it waits at most one slice, however long the
+// chains are.
 //
 // Scheduling.  One loop thread owns every device operation.  A slot has at
 // most one batch in flight.  Whenever a slot is free and requests of its
@@ -84,6 +88,12 @@ struct Request {
     // verify
     const uint8_t *packets = nullptr;
     uint64_t packets_size = 0;
+    // The device reads the packets in place, from packets_dev (the device's
+    // address of `packets`: pinned host memory), instead of the loop staging
+    // them; set by the submitter only when packets_dev and packets_size are
+    // multiples of 16.
+    bool direct = false;
+    const uint8_t *packets_dev = nullptr;
     const uint32_t *lens = nullptr, *pk = nullptr, *sig = nullptr, *msg = nullptr;
     uint8_t *out = nullptr;
     // PoH
@@ -100,6 +110,7 @@ struct Batch {
     std::vector<uint32_t> ubase;          // first signature / chain of each request
     uint32_t n = 0;                       // signatures or chains
     uint64_t packets_bytes = 0;           // verify: bytes of the packet area
+    uint32_t direct = 0;                  // verify: requests read in place
     uint64_t off = 0, size = 0;           // ring block
     uint64_t in_bytes = 0;                // bytes copied in from the block start
     uint64_t out_off = 0, out_bytes = 0;  // result area inside the block
@@ -118,22 +129,27 @@ inline uint64_t verify_block_bytes(uint64_t packets_bytes, uint64_t n) {
 inline uint64_t poh_block_bytes(uint64_t n) { return align_up(40 * n, 256); }
 
 // Packs a verify batch into its block of the ring's host copy: each request's
-// packets (16-byte aligned), then lens, pk, sig, msg tables with offsets
-// rebased into the batch; a signature whose fields fall outside its own
-// request's packets gets a signature offset past the batch (the kernel
-// rejects it).
+// packets (16-byte aligned; a request read in place leaves its area of the
+// host copy unwritten), then lens, pk, sig, msg tables with offsets rebased
+// into the batch; a signature whose fields fall outside its own request's
+// packets gets a signature offset past the batch (the kernel rejects it).
 inline void stage_verify(uint8_t *h_ring, Batch &b) {
     uint8_t *h = h_ring + b.off;
     const uint64_t tab_off = align_up(b.packets_bytes, 256);
     const uint32_t N = b.n;
     uint32_t *lens = reinterpret_cast<uint32_t *>(h + tab_off);
     uint32_t *pk = lens + N, *sig = lens + 2ull * N, *msg = lens + 3ull * N;
+    b.direct = 0;
     for (size_t k = 0; k < b.reqs.size(); k++) {
         const Request &r = *b.reqs[k];
         const uint64_t base = b.pbase[k];
-        memcpy(h + base, r.packets, r.packets_size);
-        const uint64_t pad = align_up(r.packets_size, 16) - r.packets_size;
-        if (pad) memset(h + base + r.packets_size, 0, pad);
+        if (r.direct) {
+            b.direct++;
+        } else {
+            memcpy(h + base, r.packets, r.packets_size);
+            const uint64_t pad = align_up(r.packets_size, 16) - r.packets_size;
+            if (pad) memset(h + base + r.packets_size, 0, pad);
+        }
         const uint32_t u0 = b.ubase[k];
         for (uint32_t i = 0; i < r.n; i++) {
             const uint64_t so = r.sig[i], po = r.pk[i], mo = r.msg[i], ml = r.lens[i];
@@ -148,6 +164,25 @@ inline void stage_verify(uint8_t *h_ring, Batch &b) {
     b.in_bytes = tab_off + 16ull * N;
     b.out_off = b.in_bytes;
     b.out_bytes = N;
+}
+
+// The ingest copies of a staged verify batch into its ring block, as
+// emit(source, offset in the block, bytes) calls, every one a multiple of 16
+// bytes at a 16-byte-aligned offset: each direct request's packets from its
+// own pages, and everything between them (staged packets, padding, the
+// tables) from the block's host copy h_block.  Without direct requests that
+// is one range, [0, in_bytes).
+template <class F>
+inline void ingest_ranges(const uint8_t *h_block, const Batch &b, F &&emit) {
+    uint64_t from = 0;                    // start of the host-copy range not yet emitted
+    for (size_t k = 0; b.direct && k < b.reqs.size(); k++) {
+        const Request &r = *b.reqs[k];
+        if (!r.direct || r.packets_size == 0) continue;
+        if (b.pbase[k] > from) emit(h_block + from, from, b.pbase[k] - from);
+        emit(r.packets_dev, b.pbase[k], r.packets_size);
+        from = b.pbase[k] + r.packets_size;
+    }
+    if (b.in_bytes > from) emit(h_block + from, from, b.in_bytes - from);
 }
 
 // Packs a PoH batch: the chains' 32-byte states, then their u64 counts.
@@ -191,6 +226,7 @@ struct LatencyStats {
     // seen (every device step), completion -> results delivered
     std::vector<uint64_t> stage_ns, device_ns, deliver_ns;
     uint64_t requests = 0, units = 0, batches = 0, failed = 0, steps = 0;
+    uint64_t in_place = 0;                // verify requests read in place
     void reset() { *this = LatencyStats(); }
 };
 
@@ -216,12 +252,14 @@ inline void append_class(std::string &s, const char *name, const LatencyStats &s
     char buf[1024];
     snprintf(buf, sizeof buf,
              "\"%s\": {\"requests\": %llu, \"units\": %llu, \"batches\": %llu, "
-             "\"device_steps\": %llu, \"failed\": %llu, \"p50_ms\": %.4f, \"p99_ms\": %.4f, "
+             "\"device_steps\": %llu, \"failed\": %llu, \"read_in_place\": %llu, "
+             "\"p50_ms\": %.4f, \"p99_ms\": %.4f, "
              "\"max_ms\": %.4f, \"batch_stage_ms\": [%.4f, %.4f, %.4f], "
              "\"batch_device_ms\": [%.4f, %.4f, %.4f], \"batch_deliver_ms\": [%.4f, %.4f, %.4f]}",
              name, (unsigned long long)st.requests, (unsigned long long)st.units,
              (unsigned long long)st.batches, (unsigned long long)st.steps,
-             (unsigned long long)st.failed, lat[0], lat[1], lat[2], stg[0], stg[1], stg[2],
+             (unsigned long long)st.failed, (unsigned long long)st.in_place, lat[0], lat[1],
+             lat[2], stg[0], stg[1], stg[2],
              dev[0], dev[1], dev[2], fet[0], fet[1], fet[2]);
     s += buf;
 }
@@ -408,6 +446,7 @@ class Core {
             r->state = ok ? kDone : kFailed;
             st.requests++;
             st.units += r->n;
+            if (r->direct) st.in_place++;
             if (!ok) st.failed++;
             st.ns.push_back(t - r->t_submit);
         }

@@ -9,7 +9,9 @@ Requests are cut from two pools whose expected results are known: verify
 requests are runs of consecutive packets of one validator-shaped batch, PoH
 requests are runs of chains of one chain pool.  A verify request carries its
 own packet bytes (a view into the pool) and offset tables rebased to them, as
-a caller of ed25519_verify_many would build them.
+a caller of ed25519_verify_many would build them.  With pinned pools the
+packet pool lives in pinned host memory, as solana_perf keeps its packet
+batches, so the server and the plain calls read requests in place.
 """
 import gc
 import time
@@ -20,9 +22,16 @@ import workloads as W
 
 
 class RequestPools:
-    def __init__(self, lib, verify_batch, verify_mask, poh_in, poh_counts, poh_out):
+    def __init__(self, lib, verify_batch, verify_mask, poh_in, poh_counts, poh_out,
+                 pinned=False):
         self.lib = lib
         self.b = verify_batch
+        self.pinned = pinned
+        if pinned:
+            import hip_rt
+            self.packets = hip_rt.pinned_copy(self.b.packets)
+        else:
+            self.packets = self.b.packets
         self.mask = np.asarray(verify_mask, np.uint8)
         stride = self.b.message_size
         pkt = (self.b.sig_off // stride).astype(np.int64)      # packet of each signature
@@ -47,7 +56,7 @@ class RequestPools:
         stride = self.b.message_size
         s0, s1 = int(self.first[p0]), int(self.first[p0 + npk])
         base = np.uint32(p0 * stride)
-        packets = self.b.packets[p0 * stride:(p0 + npk) * stride]
+        packets = self.packets[p0 * stride:(p0 + npk) * stride]
         return (packets, self.b.lens[s0:s1].copy(), self.b.pk_off[s0:s1] - base,
                 self.b.sig_off[s0:s1] - base, self.b.msg_off[s0:s1] - base,
                 self.mask[s0:s1])
@@ -59,10 +68,11 @@ class RequestPools:
 
 
 def build_pools(lib, seed=5, num_batches=128, poh_chains=512, poh_hashes=12500,
-                expected=None):
+                expected=None, pinned=False):
     """Pools for served load.  `expected(batch, poh_in, counts)` returns the
     verify mask and the PoH end states; by default they come from the
-    library's plain entry points (ed25519_verify_many / poh_verify_many)."""
+    library's plain entry points (ed25519_verify_many / poh_verify_many).
+    pinned: keep the packet pool in pinned host memory (hipHostMalloc)."""
     b = W.verify_validator(lib.sign, lib.create_keypair, seed=seed, num_batches=num_batches)
     poh_in, counts = W.poh_chains(poh_chains, poh_hashes, seed=seed + 1)
     if expected is None:
@@ -72,7 +82,7 @@ def build_pools(lib, seed=5, num_batches=128, poh_chains=512, poh_hashes=12500,
         assert lib.poh_verify_many(out, counts) == 0
     else:
         mask, out = expected(b, poh_in, counts)
-    return RequestPools(lib, b, mask, poh_in, counts, out)
+    return RequestPools(lib, b, mask, poh_in, counts, out, pinned=pinned)
 
 
 def log_uniform_int(rng, lo, hi):

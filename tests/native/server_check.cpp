@@ -7,9 +7,11 @@
 //
 // The core runs over a device double: one worker thread per slot plays the
 // slot's stream, and its own buffer plays the device ring.  Every step it
-// copies the batch's input bytes from the ring's host copy in (as the ingest
-// copy kernel does), computes in the device buffer, and copies the results
-// back to the host copy (the egress copy kernel), after a random delay.  The
+// copies the batch's input bytes in (as the ingest copy kernel does: from the
+// ring's host copy, and, for requests marked as read in place, from the
+// submitter's own packet buffer), computes in the device buffer, and copies
+// the results back to the host copy (the egress copy kernel), after a random
+// delay.  The
 // "kernels" are stand-ins with exact expected values:
 //   verify: out[i] = 1 + (FNV-1a of the signature, key and message bytes) % 255
 //           for a signature whose fields lie in its request's packets, else 0
@@ -17,7 +19,10 @@
 //           sliced exactly as the HIP kernels slice a chain
 // so a batch that reads another request's bytes, loses a slice, or has its
 // ring block reused while the worker still copies it gives a wrong result
-// (and, under ThreadSanitizer, a data race report).
+// (and, under ThreadSanitizer, a data race report).  Half the verify requests
+// are read in place; once such a request's wait returns, the main thread
+// overwrites its packets, which races with any read the "device" still makes
+// of them.
 //
 // Each cycle picks a ring size, batch cap, slot counts and PoH slice length,
 // then two threads submit verify and PoH requests (random sizes, offsets past
@@ -130,7 +135,13 @@ struct FakeDev {
     void run_step(const Step &s) {
         Batch &b = *s.b;
         uint8_t *d = d_ring + b.off, *h = h_ring + b.off;
-        if (s.ingest) memcpy(d, h, b.in_bytes);
+        if (s.ingest && b.kind == kVerify)
+            ingest_ranges(h, b, [d](const uint8_t *src, uint64_t off, uint64_t n) {
+                assert(off % 16 == 0 && n % 16 == 0);
+                memcpy(d + off, src, n);
+            });
+        else if (s.ingest)
+            memcpy(d, h, b.in_bytes);
         if (s.kernel && !s.fail) {
             if (b.kind == kVerify) {
                 const uint64_t tab = align_up(b.packets_bytes, 256);
@@ -244,6 +255,7 @@ struct Sub {
     Kind kind;
     int64_t ticket = 0;
     bool refused_expected = false;
+    bool direct = false;                // packets read in place by the "device"
     std::vector<uint8_t> packets, out, want;
     std::vector<uint32_t> lens, pk, sig, msg;
     std::vector<uint8_t> hashes, start;
@@ -255,7 +267,11 @@ void make_verify(Rng &r, Sub &s, uint64_t ring_cap, uint32_t max_batch) {
     const uint32_t n = r.below(10) == 0 ? 0 : 1 + (uint32_t)r.below(r.below(4) == 0 ? 300 : 24);
     const uint32_t stride = 96 + (uint32_t)r.below(200);
     const uint32_t npk = 1 + (uint32_t)r.below(n ? n : 1);
-    s.packets.resize((size_t)npk * stride + r.below(15));
+    // read in place: the size must be a multiple of 16 (the address is, as
+    // operator new aligns to 16)
+    s.direct = r.below(2) == 0;
+    const size_t bytes = (size_t)npk * stride + r.below(15);
+    s.packets.resize(s.direct ? align_up(bytes, 16) : bytes);
     for (auto &x : s.packets) x = (uint8_t)r.next();
     s.lens.resize(n);
     s.pk.resize(n);
@@ -313,6 +329,11 @@ int64_t submit(Core<FakeDev> &core, Sub &s) {
         q->sig = s.sig.data();
         q->msg = s.msg.data();
         q->out = s.out.data();
+        if (s.direct && !s.packets.empty()) {
+            assert((uintptr_t)s.packets.data() % 16 == 0);
+            q->direct = true;
+            q->packets_dev = s.packets.data();
+        }
     } else {
         q->n = (uint32_t)s.counts.size();
         q->hashes = s.hashes.data();
@@ -323,7 +344,7 @@ int64_t submit(Core<FakeDev> &core, Sub &s) {
 
 struct Counts {
     uint64_t requests = 0, refused = 0, failed = 0, verify_sigs = 0, chains = 0, steps = 0,
-             inj_step = 0, inj_launch = 0;
+             inj_step = 0, inj_launch = 0, direct = 0;
 };
 
 // A failed check ends the process at once (the loop and worker threads are
@@ -409,6 +430,10 @@ bool cycle_once(int cycle, Rng &r, Counts &c) {
             CHECK(rc != 1, "request %lld not finished within 20 s", (long long)s.ticket);
             CHECK(times[0] <= times[1] && times[1] <= times[2], "times out of order");
             if (!check_result(cycle, s, rc)) return false;
+            if (s.direct) {
+                memset(s.packets.data(), 0x5A, s.packets.size());
+                c.direct++;
+            }
             c.requests++;
             if (rc == 3) c.failed++;
             if (s.kind == kVerify) c.verify_sigs += s.want.size();
@@ -433,10 +458,12 @@ int main(int argc, char **argv) {
     for (int k = 0; k < cycles; k++)
         if (!cycle_once(k, r, c)) return 1;
     printf("cycles %d requests %llu refused %llu failed %llu signatures %llu chains %llu "
-           "steps %llu injected_step_failures %llu injected_launch_failures %llu\n",
+           "steps %llu injected_step_failures %llu injected_launch_failures %llu "
+           "read_in_place %llu\n",
            cycles, (unsigned long long)c.requests, (unsigned long long)c.refused,
            (unsigned long long)c.failed, (unsigned long long)c.verify_sigs,
            (unsigned long long)c.chains, (unsigned long long)c.steps,
-           (unsigned long long)c.inj_step, (unsigned long long)c.inj_launch);
+           (unsigned long long)c.inj_step, (unsigned long long)c.inj_launch,
+           (unsigned long long)c.direct);
     return 0;
 }

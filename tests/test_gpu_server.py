@@ -271,6 +271,75 @@ def test_mix_equals_plain_calls(gpu_lib, pools):
         assert np.array_equal(m, exp), kind
 
 
+def _bounded_mask(exp, packets_size, lens, pk, sig, msg):
+    """The mask a request gets when its packets are cut to packets_size bytes:
+    a signature whose fields leave them is rejected."""
+    ok = ((sig.astype(np.int64) + 64 <= packets_size) & (pk.astype(np.int64) + 32 <= packets_size)
+          & (msg.astype(np.int64) + lens <= packets_size))
+    return np.where(ok, exp, 0).astype(np.uint8)
+
+
+def test_pinned_requests_read_in_place(gpu_lib, pools):
+    """Verify requests whose packets sit in pinned host memory are read by the
+    device in place (a gathered copy from the callers' pages); the rest are
+    staged by the loop.  Batches mixing pageable requests, pinned ones, a
+    pinned one at an address that is not 16-byte aligned and a pinned one
+    whose size is not a multiple of 16 (both staged) give the oracle's
+    verdicts, and the server counts exactly the aligned pinned ones as read in
+    place.  A burst (many requests per batch, more than 128 copy ranges in
+    some) and then one request at a time."""
+    import hip_rt
+    stride = pools.b.message_size
+    pinned = hip_rt.pinned_copy(pools.b.packets)
+    rng = np.random.default_rng(71)
+
+    def request(kind, p0, npk):
+        packets, lens, pk, sig, msg, exp = pools.verify_request(p0, npk)
+        if kind == "pinned":
+            packets = pinned[p0 * stride:(p0 + npk) * stride]
+        elif kind == "misaligned":          # starts 8 bytes early: staged
+            packets = pinned[p0 * stride - 8:(p0 + npk) * stride]
+            pk, sig, msg = pk + 8, sig + 8, msg + 8
+        elif kind == "ragged":              # ends 4 bytes short: staged
+            packets = pinned[p0 * stride:(p0 + npk) * stride - 4]
+            exp = _bounded_mask(exp, packets.size, lens, pk, sig, msg)
+        return packets, lens, pk, sig, msg, exp
+
+    kinds = ["pageable", "pinned", "pinned", "misaligned", "ragged"]
+    server = _server(gpu_lib)
+    try:
+        for phase, count in (("burst", 300), ("one at a time", 40)):
+            reqs = []
+            for _ in range(count):
+                kind = kinds[int(rng.integers(0, len(kinds)))]
+                npk = loadgen.log_uniform_int(rng, 1, 64 if phase == "burst" else 256)
+                p0 = int(rng.integers(1, pools.num_packets - npk + 1))
+                reqs.append((kind, request(kind, p0, npk)))
+            _step(f"{phase}: {len(reqs)} verify requests")
+            want_in_place = sum(1 for k, r in reqs if k == "pinned" and r[5].size)
+            done = []
+            if phase == "burst":
+                subs = [(k, r, *server.submit_verify(*r[:5])) for k, r in reqs]
+                for k, r, t, out in subs:
+                    assert t > 0
+                    done.append((k, r, _wait_ok(server, t)[0], out))
+            else:
+                for k, r in reqs:
+                    t, out = server.submit_verify(*r[:5])
+                    assert t > 0
+                    done.append((k, r, _wait_ok(server, t)[0], out))
+            for k, r, rc, out in done:
+                assert rc == 0, k
+                assert np.array_equal(out[:r[5].size], r[5]), k
+            st = server.stats(reset=True)["verify"]
+            print(f"{phase}:", st, flush=True)
+            assert st["requests"] == len(reqs) and st["failed"] == 0
+            assert st["read_in_place"] == want_in_place > 0
+    finally:
+        _step("spl_server_destroy")
+        server.close()
+
+
 def test_config5_served_mix(gpu_lib, oracle):
     """The served mixed load at its stated shape: Poisson verify requests of
     log-uniform 1-2048 packets in the validator layout (1-4 signatures per

@@ -190,7 +190,14 @@ def test_pinned_packet_batches(gpu_lib, oracle):
             pad = hip_rt.pinned_copy(np.concatenate([np.zeros(8, np.uint8), x]))
             mixed.append(pad[8:])                             # pinned, misaligned: staged
     assert any(m.ctypes.data % 16 for m in mixed)
-    for batches in (mixed, [hip_rt.pinned_copy(x) for x in b.batches]):
+    # more pinned batches than one gathered copy launch takes (128): 2-packet
+    # pieces of the same concatenation
+    cat = np.concatenate(b.batches)
+    npk = cat.size // b.message_size
+    pieces = [hip_rt.pinned_copy(cat[i * b.message_size:min(i + 2, npk) * b.message_size])
+              for i in range(0, npk, 2)]
+    assert len(pieces) > 2 * 128
+    for batches in (mixed, [hip_rt.pinned_copy(x) for x in b.batches], pieces):
         for _ in range(3):
             got = gpu_lib.verify_many(batches, b.message_size, b.lens, b.pk_off, b.sig_off,
                                       b.msg_off)

@@ -3,8 +3,9 @@ on the CPU: tests/native/server_check.cpp drives it over a device double
 (worker threads playing the slot streams, a host buffer playing the device
 ring) through whole server lifetimes: two threads submitting verify and PoH
 requests of random sizes, offsets past their packets, zero counts, empty and
-oversized requests, a waiter with a time limit, stop() before or after the
-waits, and injected device failures (a step that fails, a launch that fails
+oversized requests, requests whose packets the "device" reads in place, a
+waiter with a time limit, stop() before or after the waits, and injected
+device failures (a step that fails, a launch that fails
 after enqueuing work).  Every result is checked exactly against the double's
 stand-in kernels.
 
@@ -57,10 +58,11 @@ def test_server_core_threadsanitizer():
     assert total["requests"] > 20000 and total["refused"] > 500
     assert total["injected_step_failures"] > 50 and total["injected_launch_failures"] > 10
     assert total["failed"] > 0 and total["chains"] > 50000 and total["signatures"] > 100000
+    assert total["read_in_place"] > 5000
 
 
 def test_server_core_optimised():
     cxx = shutil.which("g++")
     exe = _build("server_check", [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra"])
     c = _run(exe, 800, 21, 600)
-    assert c["requests"] > 30000
+    assert c["requests"] > 30000 and c["read_in_place"] > 5000
