@@ -223,7 +223,7 @@ class PerfLibs:
                           message_lens, pk_off, sk_off, msg_off, 64, use_non_default_stream)
 
     @staticmethod
-    def _many(fn, batches, message_size, lens, t1, t2, t3, out_width, nds):
+    def _prepare_many(fn, batches, message_size, lens, t1, t2, t3, out_width, nds):
         batches = [np.ascontiguousarray(b, dtype=np.uint8) for b in batches]
         elems = (GpuElems * max(1, len(batches)))()
         total_packets = 0
@@ -235,9 +235,32 @@ class PerfLibs:
         tabs = [np.ascontiguousarray(t, dtype=np.uint32) for t in (lens, t1, t2, t3)]
         n = tabs[0].size
         out = np.zeros(max(1, n * out_width), np.uint8)
-        fn(elems, len(batches), message_size, total_packets, n,
-           *[_p(t, _u32p) for t in tabs], _p(out), nds)
-        return out[: n * out_width]
+        args = (elems, len(batches), message_size, total_packets, n,
+                *[_p(t, _u32p) for t in tabs], _p(out), nds)
+        return fn, args, (batches, tabs), out[: n * out_width]
+
+    @classmethod
+    def _many(cls, fn, batches, message_size, lens, t1, t2, t3, out_width, nds):
+        fn, args, _keep, out = cls._prepare_many(fn, batches, message_size, lens, t1, t2, t3,
+                                                 out_width, nds)
+        fn(*args)
+        return out
+
+    def prepare_verify_many(self, packet_batches, message_size, message_lens, pk_off, sig_off,
+                            msg_off, use_non_default_stream=1):
+        """The arguments of one ed25519_verify_many call, converted once (the
+        gpu_Elems list, table pointers and an output buffer).  Each
+        call_prepared() of the result is then the C call alone, as a compiled
+        caller makes it; the returned mask is overwritten by the next call."""
+        return self._prepare_many(self.lib.ed25519_verify_many, packet_batches, message_size,
+                                  message_lens, pk_off, sig_off, msg_off, 1,
+                                  use_non_default_stream)
+
+    @staticmethod
+    def call_prepared(prepared):
+        fn, args, _keep, out = prepared
+        fn(*args)
+        return out
 
     def verify_packets(self, packet_batches, packet_stride, packet_sizes):
         """spl_verify_packets: wire-format packets (gpu_Elems list of whole
