@@ -82,16 +82,49 @@ def note(*a):
 
 def poh_issued_per_hash(kernel):
     """Issued VALU lane-ops per chain-hash of a PoH kernel (SQ_INSTS_VALU x 64
-    per hash step of 64 chains, both waves of the split kernel summed), as
-    recorded in profiles/verify_poh_sign_bench.md; None when absent."""
-    path = os.path.join(ROOT, "profiles", "verify_poh_sign_bench.md")
+    lanes / chains / hashes, every wave of the kernel counted), as recorded in
+    profiles/poh_kernels.md; None when absent."""
+    path = os.path.join(ROOT, "profiles", "poh_kernels.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 6 and cells[0] == kernel and "per 64-chain hash" in cells[4]:
+        if len(cells) >= 6 and cells[0] == kernel and "lane-ops per chain-hash" in cells[4]:
             return float(cells[4].split()[0].replace(",", ""))
     return None
+
+
+POH_WORKLOADS = (("c2_4096x12500", 4096, 12500), ("sat_262144x500", 262144, 500))
+
+
+def poh_leg(lib, stream):
+    """PoH on the device: the plain 4096 x 12,500 workload and a saturating
+    one, each timed as one poh_verify_many_device launch (HIP events on its
+    stream) after one untimed launch, in the kernel the library picks for that
+    many chains."""
+    import hip_rt
+    import workloads as W
+    poh = {}
+    for name, n, reps in POH_WORKLOADS:
+        h, counts = W.poh_chains(n, reps, seed=2)
+        dh = hip_rt.DeviceBuffer.from_array(h)
+        dc = hip_rt.DeviceBuffer.from_array(counts)
+        lib.poh_verify_many_device(dh.ptr, dc.ptr, n, stream.handle)
+        stream.synchronize()
+        e0, e1 = hip_rt.Event(), hip_rt.Event()
+        e0.record(stream)
+        lib.poh_verify_many_device(dh.ptr, dc.ptr, n, stream.handle)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1)
+        hps = n * reps / (ms / 1e3)
+        kern = f"poh_{lib.poh_kernel_choice(n)}_kernel"
+        per = poh_issued_per_hash(kern)
+        poh[name] = {"hashes_per_s": hps, "ms": ms, "kernel": kern,
+                     "issued_frac": None if per is None else hps * per / VALU_PEAK,
+                     "ref_equiv_frac": hps * W_SHA256 / VALU_PEAK}
+        log(f"bench: poh {name} ({kern}): {hps:.3e} hashes/s")
+    return poh
 
 
 def cpu_threads():
@@ -539,6 +572,8 @@ def main():
                     help="run only the served-load leg (with its request pools)")
     ap.add_argument("--served-pinned", action="store_true",
                     help="with --served-only: the requests' packets in pinned host memory")
+    ap.add_argument("--poh-only", action="store_true",
+                    help="run only the PoH leg and print its figures")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before timing "
                          "(a known-size dispatch that calibrates HBM byte counters)")
@@ -555,6 +590,9 @@ def main():
     lib = perf_libs.load()
     assert lib.init(), "no usable GPU"
     stream = hip_rt.Stream()
+    if args.poh_only:
+        print(json.dumps({"poh": poh_leg(lib, stream)}), flush=True)
+        return
 
     log("bench: generating validator-shaped batch")
     b = W.verify_validator(lib.sign, lib.create_keypair, seed=3)
@@ -682,28 +720,7 @@ def main():
     eng.close()
 
     if not args.no_extra:
-        # PoH: the plain 4096 x 12,500 workload and a saturating one
-        poh = {}
-        for name, n, reps in (("c2_4096x12500", 4096, 12500), ("sat_262144x500", 262144, 500)):
-            h, counts = W.poh_chains(n, reps, seed=2)
-            dh = hip_rt.DeviceBuffer.from_array(h)
-            dc = hip_rt.DeviceBuffer.from_array(counts)
-            lib.poh_verify_many_device(dh.ptr, dc.ptr, n, stream.handle)
-            stream.synchronize()
-            e0, e1 = hip_rt.Event(), hip_rt.Event()
-            e0.record(stream)
-            lib.poh_verify_many_device(dh.ptr, dc.ptr, n, stream.handle)
-            e1.record(stream)
-            e1.synchronize()
-            ms = e0.elapsed_ms(e1)
-            hps = n * reps / (ms / 1e3)
-            kern = "poh_split_kernel" if n <= 32768 else "poh_chain_kernel"
-            per = poh_issued_per_hash(kern)
-            poh[name] = {"hashes_per_s": hps, "ms": ms, "kernel": kern,
-                         "issued_frac": None if per is None else hps * per / VALU_PEAK,
-                         "ref_equiv_frac": hps * W_SHA256 / VALU_PEAK}
-            log(f"bench: poh {name}: {hps:.3e} hashes/s")
-        result["poh"] = poh
+        result["poh"] = poh_leg(lib, stream)
         # plain ABI call latency (host buffers, PCIe both ways)
         p = W.verify_plain(1024, lib.sign, lib.create_keypair, seed=1)
         lat = []

@@ -126,6 +126,20 @@ int ed25519_sign_many_device(const uint8_t *d_packets, uint64_t packets_size,
 int poh_verify_many_device(uint8_t *d_hashes, const uint64_t *d_num_hashes, size_t num_elems,
                            void *stream);
 
+/* No counterpart in the reference.  poh_verify_many_device with the kernel
+ * named instead of chosen by chain count, for tests and tuning: 0 = the
+ * automatic choice, 1 = one chain per lane, 2 = rounds and message schedule
+ * in two waves, 3 = as 2 with two lanes per chain in both waves.  Every kernel
+ * gives the same end states.  Returns as poh_verify_many_device; 2 also for an
+ * unknown kernel. */
+int spl_poh_device_kernel(uint8_t *d_hashes, const uint64_t *d_num_hashes, size_t num_elems,
+                          int kernel, void *stream);
+
+/* No counterpart in the reference.  The kernel (numbered as above, 1-3) that
+ * poh_verify_many and poh_verify_many_device use for num_elems chains on the
+ * current device; 0 when no GPU is usable. */
+int spl_poh_kernel_choice(size_t num_elems);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): ed25519_verify_many_device with the same arguments,

@@ -202,6 +202,10 @@ def copy_d2d(dst, src, nbytes, stream=None):
           "hipMemcpyAsync D2D")
 
 
+# hipDeviceAttribute_t values (hip/hip_runtime_api.h, ROCm 7)
+HIP_ATTR_CUS = 63   # hipDeviceAttributeMultiprocessorCount
+
+
 def device_attribute(attr, device=0):
     v = ctypes.c_int(0)
     check(_hip.hipDeviceGetAttribute(ctypes.byref(v), attr, device), "hipDeviceGetAttribute")

@@ -32,6 +32,7 @@ EXPORTS = (
     "ed25519_get_checked_scalar", "ed25519_check_packed_ge_small_order",
     "ed25519_free_gpu_mem", "poh_verify_many_set_verbose", "poh_verify_many",
     "ed25519_verify_many_device", "ed25519_sign_many_device", "poh_verify_many_device",
+    "spl_poh_device_kernel", "spl_poh_kernel_choice",
     "ed25519_verify_many_device_noalloc",
     "spl_verify_engine_create", "spl_verify_engine_destroy", "spl_verify_engine_buffers",
     "spl_verify_engine_create_pipelined", "spl_verify_engine_depth",
@@ -99,6 +100,10 @@ class PerfLibs:
         lib.ed25519_verify_many_device_noalloc.restype = ctypes.c_int
         lib.poh_verify_many_device.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
         lib.poh_verify_many_device.restype = ctypes.c_int
+        lib.spl_poh_device_kernel.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]
+        lib.spl_poh_device_kernel.restype = ctypes.c_int
+        lib.spl_poh_kernel_choice.argtypes = [ctypes.c_size_t]
+        lib.spl_poh_kernel_choice.restype = ctypes.c_int
         lib.spl_verify_engine_create.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
         lib.spl_verify_engine_create.restype = _vp
         lib.spl_verify_engine_create_pipelined.argtypes = [ctypes.c_uint32, ctypes.c_uint64,
@@ -324,6 +329,21 @@ class PerfLibs:
 
     def poh_verify_many_device(self, d_hashes, d_counts, n, stream=0):
         return self.lib.poh_verify_many_device(d_hashes, d_counts, n, stream)
+
+    # kernel names for poh_device_kernel (spl_poh_device_kernel's numbering)
+    POH_KERNELS = {"auto": 0, "chain": 1, "split": 2, "pair": 3}
+
+    def poh_device_kernel(self, d_hashes, d_counts, n, kernel, stream=0):
+        """poh_verify_many_device with the kernel named ("chain", "split",
+        "pair" or "auto")."""
+        return self.lib.spl_poh_device_kernel(d_hashes, d_counts, n, self.POH_KERNELS[kernel],
+                                              stream)
+
+    def poh_kernel_choice(self, n):
+        """Name of the PoH kernel the library uses for n chains ("chain",
+        "split", "pair"), or None without a GPU."""
+        k = self.lib.spl_poh_kernel_choice(n)
+        return next((name for name, v in self.POH_KERNELS.items() if v == k and k), None)
 
     def packet_offsets_device(self, d_packets, num, stride, d_sizes, max_sigs, d_sig_lens,
                               d_sig_base, d_lens, d_pk, d_sig, d_msg, d_counts, stream=0):

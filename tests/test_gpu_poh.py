@@ -80,16 +80,29 @@ def test_device_resident_form(gpu_lib, oracle):
     assert np.array_equal(dh.download(), want)
 
 
-def test_both_kernels_vs_oracle(gpu_lib, oracle):
-    """Up to 32 chains per SIMD (32,768 on 256 CUs) run on two waves per 64
-    chains (rounds and message schedule split); more run one chain per lane.
-    Both sides of the switch, with ragged counts including zeros, and a group
-    whose one long chain makes the other 63 idle."""
+def _simds():
+    import hip_rt
+    return 4 * hip_rt.device_attribute(hip_rt.HIP_ATTR_CUS)
+
+
+def _ragged(n, rng, seed):
+    h, _ = W.poh_chains(n, 0, seed=seed)
+    counts = rng.integers(0, 24, n).astype(np.uint64)
+    counts[rng.random(n) < 0.05] = 0
+    return h, counts
+
+
+def test_kernel_switches_vs_oracle(gpu_lib, oracle):
+    """poh_verify_many picks the pair kernel up to 8 chains per SIMD, the split
+    kernel up to 32 and one chain per lane beyond: both sides of each switch,
+    with ragged counts including zeros, and a group whose one long chain makes
+    the others idle (they keep their end states)."""
     rng = np.random.default_rng(14)
-    for n in (32768, 32769, 40000):
-        h, _ = W.poh_chains(n, 0, seed=n)
-        counts = rng.integers(0, 24, n).astype(np.uint64)
-        counts[rng.random(n) < 0.05] = 0
+    simds = _simds()
+    for n, kernel in ((8 * simds, "pair"), (8 * simds + 1, "split"), (32 * simds, "split"),
+                      (32 * simds + 1, "chain"), (40 * simds, "chain")):
+        assert gpu_lib.poh_kernel_choice(n) == kernel, n
+        h, counts = _ragged(n, rng, seed=n)
         counts[64:128] = 0
         counts[100] = 300
         want = h.copy()
@@ -98,3 +111,33 @@ def test_both_kernels_vs_oracle(gpu_lib, oracle):
         assert gpu_lib.poh_verify_many(got, counts) == 0
         assert np.array_equal(got, want), n
         assert np.array_equal(got[32 * 64:32 * 100], h[32 * 64:32 * 100])
+
+
+@pytest.mark.parametrize("kernel", ["chain", "split", "pair"])
+def test_every_kernel_vs_oracle(gpu_lib, oracle, kernel):
+    """Each PoH kernel, named through spl_poh_device_kernel, on chain counts
+    around its group sizes (32 chains for the pair kernel, 64 for the others),
+    ragged counts with zeros, one chain far longer than its group, and the
+    full 4096-chain shape at 2,000 hashes: every end state equals the
+    oracle's."""
+    import hip_rt
+    rng = np.random.default_rng(15)
+    s = hip_rt.Stream()
+    for n in (1, 2, 31, 32, 33, 63, 64, 65, 100, 2049):
+        h, counts = _ragged(n, rng, seed=1000 + n)
+        counts[n // 2] = 61
+        want = h.copy()
+        oracle.poh_verify_many(want, counts, threads=8)
+        dh = hip_rt.DeviceBuffer.from_array(h)
+        dc = hip_rt.DeviceBuffer.from_array(counts)
+        assert gpu_lib.poh_device_kernel(dh.ptr, dc.ptr, n, kernel, s.handle) == 0
+        s.synchronize()
+        assert np.array_equal(dh.download(), want), (kernel, n)
+    h, counts = W.poh_chains(4096, 2000, seed=16)
+    want = h.copy()
+    oracle.poh_verify_many(want, counts, threads=16)
+    dh = hip_rt.DeviceBuffer.from_array(h)
+    dc = hip_rt.DeviceBuffer.from_array(counts)
+    assert gpu_lib.poh_device_kernel(dh.ptr, dc.ptr, 4096, kernel, s.handle) == 0
+    s.synchronize()
+    assert np.array_equal(dh.download(), want), kernel

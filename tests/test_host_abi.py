@@ -125,6 +125,13 @@ out = L.verify_many([pkt], 256, [100], [64], [0], [96])
 assert out.tolist() == [0]
 assert L.lib.spl_server_create(0, 0, 0, 0) is None
 assert L.lib.spl_verify_engine_create(16, 4096) is None
+# PoH kernel hook: an unknown kernel is refused before any device work; a
+# known one reports no GPU
+assert L.lib.spl_poh_device_kernel(None, None, 4, 4, None) == 2
+assert L.lib.spl_poh_device_kernel(None, None, 4, -1, None) == 2
+assert L.lib.spl_poh_device_kernel(None, None, 0, 3, None) == 0
+assert L.lib.spl_poh_device_kernel(16, 16, 4, 3, None) == 1
+assert L.poh_kernel_choice(4096) is None
 try:
     perf_libs.Server(L)
     raise SystemExit('Server() must raise without a GPU')
