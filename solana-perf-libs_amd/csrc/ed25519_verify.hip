@@ -38,13 +38,10 @@ VerifyConfig tuned_plain_config(int dev, uint32_t n) {
             if (e.valid) return e.best_plain;
         }
     }
-    // Untuned: a launch that fits one wave per SIMD is bound by one lane's
-    // latency, which two lanes per signature shorten (measured 0.69-0.75 ms
-    // against 0.93-0.98 ms for 512-16384 signatures); beyond that the pair
-    // kernel's extra total work costs more than it saves.
-    const uint32_t simds = 4u * (uint32_t)device_cus(dev);
-    if (2ull * n <= 64ull * simds) return kPairVerifyConfig;
-    return kDefaultVerifyConfig;
+    // Untuned: a launch that fits one wave per SIMD is bound by one
+    // signature's latency, which the staged and two-lane forms shorten;
+    // beyond that their extra waves cost more than they save.
+    return untuned_plain_config(n, 4u * (uint32_t)device_cus(dev));
 }
 
 int device_cus(int dev) {

@@ -20,11 +20,27 @@ struct VerifyConfig {
                           //    [y S]B + [x](+-A) + [y](-R) = O, about 128-bit x, y
     uint8_t lanes;        // lanes per signature: 1, or 2 (half only, one-shot only):
                           //    one lane runs the x(+-A) chain, its neighbour the
-                          //    y(-R) chain, each with half of the comb digits
+                          //    y(-R) chain, each with half of the comb digits;
+                          //    3 (half only, one-shot only): the staged form, the
+                          //    two chain lanes in one wave and the hashing, the
+                          //    lattice step and the comb in a second wave
 };
 
 constexpr VerifyConfig kDefaultVerifyConfig = {0, 4, 1, 0, 0, 1, 1};
 constexpr VerifyConfig kPairVerifyConfig = {0, 4, 1, 0, 0, 1, 2};
+constexpr VerifyConfig kStagedVerifyConfig = {0, 5, 1, 0, 0, 1, 3};
+constexpr VerifyConfig kStagedVerifyConfigW4 = {0, 4, 1, 0, 0, 1, 3};
+
+// The untuned one-shot choice for n signatures on a device with `simds`
+// SIMDs: the staged form while its two waves per 32 signatures get a SIMD
+// each, two lanes per signature while those waves do, else one lane.
+// Measured on 512-16,384 validator-shaped signatures (256 CUs): staged
+// 0.48-0.49 ms, two lanes 0.64-0.66 ms, one lane 0.88-0.89 ms per launch.
+inline VerifyConfig untuned_plain_config(uint32_t n, uint32_t simds) {
+    if (16ull * simds >= n) return kStagedVerifyConfig;
+    if (32ull * simds >= n) return kPairVerifyConfig;
+    return kDefaultVerifyConfig;
+}
 
 // Device-side description of one verify batch.  All pointers are device
 // pointers; n lives in device memory so a captured graph can be replayed for
@@ -63,9 +79,7 @@ int verify_config_candidates(VerifyConfig *out, int max);
 void config_name(const VerifyConfig &c, char *buf, size_t len);
 
 // The fastest one-shot configuration the autotuner recorded for n's size
-// class on HIP device dev; when none was recorded, two lanes per signature
-// (kPairVerifyConfig) if the launch fits one wave per SIMD, else
-// kDefaultVerifyConfig.
+// class on HIP device dev; when none was recorded, untuned_plain_config.
 VerifyConfig tuned_plain_config(int dev, uint32_t n);
 
 // Compute units of HIP device dev (256 if the query fails).

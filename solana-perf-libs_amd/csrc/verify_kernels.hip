@@ -222,6 +222,173 @@ verify_pair_kernel(VerifyJob job, uint32_t n_arg) {
     verify_sig_pair<W>(job, i, g & 1u, job.scratch + (size_t)g * VarWindow<W>::kTable);
 }
 
+// Staged form: the two-lane form with the stages that do not need the points
+// moved to a second wave.  A workgroup covers 32 signatures with two waves:
+//   hash wave  (lanes 0..31, one per signature): the byte-level checks, the
+//              SHA-512 of R || A || M, h mod L, the half scalars x, y and
+//              sp = y S mod L, then the whole fixed-base sum [sp]B (32 comb
+//              additions);
+//   point wave (two lanes per signature, as in the two-lane form): decode
+//              -A or -R and build its table of multiples, then its doubling
+//              chain [|x|](+-A) or [y](-R) with no comb additions.
+// The two waves hand over through LDS at two barriers: x, y and the checks
+// once the point wave has its tables (it decodes while the hash wave
+// hashes), and [sp]B once the chains are done.  A signature's critical path
+// becomes max(hash + lattice, decode + table) + chain, against their sum in
+// the two-lane form, and no stage runs twice.  Every lane of both waves
+// reaches both barriers (rejected and out-of-range signatures compute on and
+// write nothing), so the barriers sit in control flow that both waves run.
+struct StagedShared {
+    uint32_t x[8][32], y[8][32];   // |x| and y, by word
+    uint32_t flags[32];            // bit 0: passed the byte checks, bit 1: x < 0
+    int32_t comb[40][32];          // [sp]B as a ge_cached, by limb
+};
+
+template <int W>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
+verify_staged_kernel(VerifyJob job, uint32_t n_arg) {
+    raise_wave_priority();
+    __shared__ StagedShared sh;
+    const ed25519_tables &T = *job.T;
+    const uint32_t n = job.n_dev ? *job.n_dev : n_arg;
+    // a graph captured for a larger capacity launches workgroups past the
+    // batch: both waves of such a group leave here, before either barrier
+    if (blockIdx.x * 32u >= n) return;
+    const uint32_t lane = threadIdx.x & 63;
+    constexpr int kT = VarWindow<W>::kTable;
+    if (threadIdx.x < 64) {
+        // ---- hash wave
+        const uint32_t j = lane, i = blockIdx.x * 32u + lane;
+        const bool mine = j < 32 && i < n;
+        uint32_t x[8] = {0, 0, 0, 0, 0, 0, 0, 0}, y[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t sp[8] = {0, 0, 0, 0, 0, 0, 0, 0}, x_neg = 0, ok = 0;
+        if (mine) {
+            const uint64_t so = job.sig_off[i], po = job.pk_off[i], mo = job.msg_off[i],
+                           ml = job.lens[i];
+            if (so + 64 <= job.packets_size && po + 32 <= job.packets_size &&
+                mo + ml <= job.packets_size) {
+                uint32_t sig[16], pk[8];
+                load_words(sig, job.packets + so, 16);
+                load_words(pk, job.packets + po, 8);
+                MsgReader msg{job.packets + mo, (uint32_t)ml};
+                ok = !(sig[15] & 0xf0000000u) && enc_y_canonical(sig) && !y_low_order(sig) &&
+                     !y_low_order(pk);
+                uint32_t pre[16];
+#pragma unroll
+                for (int k = 0; k < 8; k++) { pre[k] = sig[k]; pre[8 + k] = pk[k]; }
+                uint64_t st[8];
+                sha512_prefixed(st, pre, 16, msg);
+                uint32_t hw[16], h[8];
+                sha512_digest_le_words(hw, st);
+                sc_reduce512(h, hw);
+                half_scalars(h, x, x_neg, y);
+                const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                sc_muladd(sp, y, sig + 8, zero);
+            }
+        }
+        if (j < 32) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) { sh.x[k][j] = x[k]; sh.y[k][j] = y[k]; }
+            sh.flags[j] = ok | (x_neg << 1);
+        }
+        __syncthreads();
+        // [sp]B from the comb while the point wave runs its chains
+        ge_p3 pb;
+        ge_p3_0(pb);
+        if (j < 32) {
+            ge_p1p1 tb;
+            uint32_t below = 0;
+            for (int m = 0; m < 32; m++) {
+                const int df = take_low_byte_digit(sp, below);
+                ge_add_precomp(tb, pb, T.comb[m][df < 0 ? -df : df], df < 0);
+                ge_p1p1_to_p3(pb, tb);
+            }
+            ge_cached cb;
+            ge_p3_to_cached(cb, pb, T.k.d2);
+            const int32_t *cw = reinterpret_cast<const int32_t *>(&cb);
+#pragma unroll
+            for (int k = 0; k < 40; k++) sh.comb[k][j] = cw[k];
+        }
+        __syncthreads();
+        return;
+    }
+    // ---- point wave: lanes 2j (-A) and 2j + 1 (-R) of signature j
+    const uint32_t j = lane >> 1, role = lane & 1u, i = blockIdx.x * 32u + j;
+    const bool valid = i < n;
+    // out-of-range lanes read (never write) the first lane's table of the group
+    ge_cached *tab = job.scratch + (size_t)(blockIdx.x * 64u + (valid ? lane : 0u)) * kT;
+    uint32_t enc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (valid) {
+        const uint64_t off = role ? job.sig_off[i] : job.pk_off[i];
+        if (off + 32 <= job.packets_size) load_words(enc, job.packets + off, 8);
+    }
+    ge_p3 P;
+    int dec = ge_decode_neg(P, enc, T.k) == 0;      // -A or -R
+    dec &= pair_swap(dec);
+    if (valid) build_multiples(tab, kT, P, T.k.d2);
+    __syncthreads();
+    uint32_t a[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = role ? sh.y[k][j] : sh.x[k][j];
+    const uint32_t flags = sh.flags[j];
+    // the A lane's table holds multiples of -A; for x < 0 it needs +A
+    const uint32_t flip = role ? 0u : (flags >> 1) & 1u;
+    int bits = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (a[k]) bits = 32 * k + bitlen32(a[k]);
+    const int K = wave_max7(half_windows<W>(bits));
+    shl256(a, 256 - K * W);
+    ge_p3 u;
+    ge_p1p1 t;
+    ge_p2 r;
+    ge_p2_0(r);
+    for (int w = K - 1; w >= 0; w--) {
+        const int d = take_top_digit<W>(a);
+        if (w != K - 1) {
+#pragma unroll
+            for (int k = 0; k < W - 1; k++) {
+                ge_p2_dbl(t, r);
+                ge_p1p1_to_p2(r, t);
+            }
+            ge_p2_dbl(t, r);
+            ge_p1p1_to_p3(u, t);
+        } else {
+            ge_p3_0(u);
+        }
+        ge_add_cached(t, u, tab[d < 0 ? -d : d], (uint32_t)(d < 0) ^ flip);
+        if (w != 0) ge_p1p1_to_p2(r, t);
+    }
+    ge_p1p1_to_p3(u, t);
+    __syncthreads();
+    if (!role) {
+        // the A lane adds the fixed-base part
+        ge_cached cb;
+        int32_t *cw = reinterpret_cast<int32_t *>(&cb);
+#pragma unroll
+        for (int k = 0; k < 40; k++) cw[k] = sh.comb[k][j];
+        ge_add_cached(t, u, cb, 0);
+        ge_p1p1_to_p3(u, t);
+    }
+    // add the partner's part
+    ge_p3 v;
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        v.X.v[k] = pair_swap(u.X.v[k]);
+        v.Y.v[k] = pair_swap(u.Y.v[k]);
+        v.Z.v[k] = pair_swap(u.Z.v[k]);
+        v.T.v[k] = pair_swap(u.T.v[k]);
+    }
+    ge_cached c;
+    ge_p3_to_cached(c, v, T.k.d2);
+    ge_add_cached(t, u, c, 0);
+    ge_p1p1_to_p2(r, t);
+    fe dz;
+    fe_sub(dz, r.Y, r.Z);
+    const uint32_t id = fe_iszero(r.X) & fe_iszero(dz);
+    if (valid && !role) job.out[i] = (uint8_t)(id & (uint32_t)dec & (flags & 1u));
+}
+
 template <int W, int WPE, int HALF>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 verify_persistent_kernel(VerifyJob job) {
@@ -327,6 +494,11 @@ static void launch_pair(const VerifyJob &job, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL((verify_pair_kernel<W, WPE>), dim3((n + 31) / 32), dim3(64), 0, s, job, n);
 }
 
+template <int W>
+static void launch_staged(const VerifyJob &job, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL((verify_staged_kernel<W>), dim3((n + 31) / 32), dim3(128), 0, s, job, n);
+}
+
 template <int HALF>
 static void launch_plain_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint32_t n,
                              hipStream_t s) {
@@ -343,6 +515,10 @@ hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint
                              int num_cus, hipStream_t s) {
     if (n_grid == 0) return hipSuccess;
     const bool w5 = cfg.window == 5;
+    if (!cfg.persistent && cfg.lanes == 3) {
+        w5 ? launch_staged<5>(job, n_grid, s) : launch_staged<4>(job, n_grid, s);
+        return hipGetLastError();
+    }
     if (!cfg.persistent && cfg.lanes == 2) {
         const bool wpe2 = cfg.waves_per_eu == 2;
         if (w5) wpe2 ? launch_pair<5, 2>(job, n_grid, s) : launch_pair<5, 1>(job, n_grid, s);
@@ -373,7 +549,9 @@ hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint
 }
 
 void config_name(const VerifyConfig &c, char *buf, size_t len) {
-    if (c.lanes == 2)
+    if (c.lanes == 3)
+        snprintf(buf, len, "one-shot W=%u scalars=half staged", c.window);
+    else if (c.lanes == 2)
         snprintf(buf, len, "one-shot W=%u scalars=half lanes=2 waves_per_eu=%u", c.window,
                  c.waves_per_eu);
     else if (c.persistent)
@@ -401,6 +579,8 @@ int verify_config_candidates(VerifyConfig *out, int max) {
     for (int w = 4; w <= 5; w++)
         for (int wpe = 1; wpe <= 2; wpe++)
             if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, (uint8_t)wpe, 0, 0, 1, 2};
+    for (int w = 4; w <= 5; w++)
+        if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, 1, 0, 0, 1, 3};
     return k;
 }
 

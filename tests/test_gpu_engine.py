@@ -227,6 +227,33 @@ def test_every_configuration(gpu_lib, oracle):
         eng.close()
 
 
+def test_staged_form_sizes(gpu_lib, oracle):
+    """The staged form (hash wave + point wave per 32 signatures, handing over
+    through LDS at two barriers) on batch sizes around its groups: a partly
+    filled last group, and groups wholly past the batch in a graph captured
+    for the engine's capacity (they leave before either barrier)."""
+    import perf_libs
+    b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=27, num_batches=12,
+                           num_keys=64, flip_frac=0.1, noncanon_frac=0.03,
+                           small_order_frac=0.03)
+    want = oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off, threads=8)
+    eng = perf_libs.VerifyEngine(gpu_lib, b.n, b.packets.size)
+    try:
+        _load(eng, b)
+        names = eng.candidates()
+        staged = [k for k, nm in enumerate(names) if "staged" in nm]
+        assert len(staged) == 2, names
+        for k in staged:
+            eng.use(k)
+            for n in (1, 2, 31, 32, 33, 64, 65, 500, b.n - 1, b.n):
+                assert eng.run(n) == 0 and eng.sync() == 0
+                got = _mask(eng, n)
+                assert np.array_equal(got, want[:n]), (names[k], n,
+                                                       np.nonzero(got != want[:n])[0][:10])
+    finally:
+        eng.close()
+
+
 def _fixture_batch(rows, stride=512):
     """The rows of a verify fixture as one batch, one signature per packet:
     [0,64) signature, [64,96) key, [96,96+len) message."""
