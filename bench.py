@@ -326,7 +326,7 @@ def abi_callers_leg(lib, b, mask, calls=8, thread_counts=(1, 2, 4)):
     return out
 
 
-SERVED_RATES = (1e5, 2e5, 5e5, 1e6, 2e6, 4e6, 8e6, 12e6, 14e6, 16e6, 20e6, 24e6)
+SERVED_RATES = (1e5, 2e5, 5e5, 1e6, 2e6, 4e6, 8e6, 12e6, 14e6, 16e6, 20e6, 22e6, 24e6)
 
 
 def served_leg(lib, b, mask, seconds=1.0, progress=lambda: None, partial=None, slots=(0, 0),
