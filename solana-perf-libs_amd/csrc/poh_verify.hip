@@ -355,15 +355,15 @@ struct PairSchedule {
     uint32_t q1, q2, q3;   // sigma counts: e (sigma0) 7, 18, >>3; a (sigma1) 17, 19, >>10
 };
 
-// W[t] for t in [T0, T0 + 16).  Ring w: on an e lane w[s] = W[j] with
-// j = s (mod 16), on an a lane j = s + 13 (mod 16).  kwv[t - T0] receives
+// W[t] for t in [T0, T1).  Ring w: on an e lane w[s] = W[j] with
+// j = s (mod 16), on an a lane j = s + 13 (mod 16).  kwv[t - 16] receives
 // K[t] + W[t] (meaningful on the e lanes).
-template <int T0>
-__device__ __forceinline__ void pair_schedule16(uint32_t (&w)[16], uint32_t (&kwv)[16],
-                                                const PairSchedule &c) {
+template <int T0, int T1>
+__device__ __forceinline__ void pair_schedule(uint32_t (&w)[16], uint32_t (&kwv)[48],
+                                              const PairSchedule &c) {
     const uint32_t K[64] = SPL_SHA256_K;
 #pragma unroll
-    for (int t = T0; t < T0 + 16; t++) {
+    for (int t = T0; t < T1; t++) {
         const uint32_t x = w[(t - 15) & 15];   // e: W[t-15], a: W[t-2]
         const uint32_t sg = xor3_32(__builtin_amdgcn_alignbit(x, x, c.q1),
                                     __builtin_amdgcn_alignbit(x, x, c.q2), x >> c.q3);
@@ -374,21 +374,28 @@ __device__ __forceinline__ void pair_schedule16(uint32_t (&w)[16], uint32_t (&kw
         w[t & 15] = wt;
         // a slot t+3 takes W[t]; e slot t+3 keeps W[t-13]
         w[(t + 3) & 15] = from_e_lane(w[(t + 3) & 15], wt);
-        kwv[t - T0] = wt + K[t];
+        kwv[t - 16] = wt + K[t];
     }
 }
 
-// The rounds wave's phases end after rounds 15, 31 and 47 (round 64 ends the
-// fourth).  Each of the schedule wave's blocks (W16..31, W32..47, W48..63) is
-// loaded right after the barrier that publishes it, one round before its
-// first word is used.  Timed on 4096 x 12,500: 0.6-2.4 % faster than ends at
-// 15/30/46, 15/30/45, 14/29/45, 15/29/46 or 14/30/46 (which give the loads a
-// second round), and with all three ends moved together (P/P+16/P+32), 0.5 %
-// faster than P = 16 and 2-6 % faster than P = 11-14.
+// The schedule wave writes K + W in three blocks, W[16..B1), W[B1..B2) and
+// W[B2..64), one per phase; the rounds wave's phases end after rounds P1, P2
+// and P3 (round 64 ends the fourth) and it loads each block right after the
+// barrier that publishes it.  Round t of phase k + 1 may only use a block the
+// schedule wrote in phases 1..k, hence P2 <= B1 and P3 <= B2; rounds 16 - P1,
+// B1 - P2 and B2 - P3 run from registers while the loads are in flight.
+// Timed on 4096 x 12,500 (20.66 ms here): blocks ending at 31/47, 30/46,
+// 31/46 or 30/47 with phase ends two rounds before them are 1.3-4 % slower,
+// and so are phase ends 15/30/46, 14/29/45, 14/30/46 and others with these
+// blocks (0.6-2.4 %), and P/P+16/P+32 for P = 11-14 or 16 (0.5-6 %): a second
+// round of cover for the loads does not pay for the rounds it moves.
+constexpr int kPohPairBlockEnd[2] = {32, 48};
 constexpr int kPohPairPhaseEnd[3] = {15, 31, 47};
 static_assert(kPohPairPhaseEnd[0] >= 8 && kPohPairPhaseEnd[0] <= 16 &&
-                  kPohPairPhaseEnd[1] >= 16 && kPohPairPhaseEnd[1] <= 32 &&
-                  kPohPairPhaseEnd[2] >= 32 && kPohPairPhaseEnd[2] <= 48,
+                  kPohPairPhaseEnd[1] >= 16 && kPohPairPhaseEnd[1] <= kPohPairBlockEnd[0] &&
+                  kPohPairPhaseEnd[2] >= kPohPairBlockEnd[0] &&
+                  kPohPairPhaseEnd[2] <= kPohPairBlockEnd[1] && kPohPairBlockEnd[0] > 16 &&
+                  kPohPairBlockEnd[1] > kPohPairBlockEnd[0] && kPohPairBlockEnd[1] < 64,
               "phase 1 holds the message rounds; no round runs before its block is loaded");
 
 __global__ void __launch_bounds__(kPohPairThreads)
@@ -450,19 +457,17 @@ poh_pair_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ coun
     for (int t = 0; t < 4; t++) ys[t] = pl.a ? kStartA.y[t] : kStartE.y[t];
     const uint32_t p1c = pl.a ? kStartA.p1 : kStartE.p1, q1c = pl.a ? kStartA.q1 : kStartE.q1;
     uint32_t r1 = 0, r2 = 0, r3 = 0, r4 = 0;
-    uint32_t w[16], kwv[16];
-    uint32_t k0[8], qa[16], qb[16], qc[16];
+    uint32_t w[16], kwv[48];
+    uint32_t k0[8], q[48];
     // K[t] + W[t] of round t: message words (k0), constants (ke), then the
-    // schedule wave's three blocks of 16, each loaded right after the barrier
-    // that publishes it.  t is a constant in every unrolled use.
-    auto kwt = [&](int t) -> uint32_t {
-        return t < 8 ? k0[t] : t < 16 ? ke[t] : t < 32 ? qa[t - 16] : t < 48 ? qb[t - 32]
-                                                                         : qc[t - 48];
-    };
+    // schedule wave's three blocks, each loaded right after the barrier that
+    // publishes it.  t is a constant in every unrolled use.
+    auto kwt = [&](int t) -> uint32_t { return t < 8 ? k0[t] : t < 16 ? ke[t] : q[t - 16]; };
     constexpr int P1 = kPohPairPhaseEnd[0], P2 = kPohPairPhaseEnd[1], P3 = kPohPairPhaseEnd[2];
+    constexpr int B1 = kPohPairBlockEnd[0], B2 = kPohPairBlockEnd[1];
     const uint32_t srow = pl.a ? 13 : 0;   // schedule ring rotation of this lane
     for (uint64_t r = 0; r < reps; r++) {
-        // phase 1: rounds 0..P1-1 | W16..31
+        // phase 1: rounds 0..P1-1 | W[16..B1)
         if (rounds) {
 #pragma unroll
             for (int t = 4; t < 8; t++) k0[t] = add_e_word<false>(ke[t], dg[t - 4]);
@@ -483,47 +488,47 @@ poh_pair_kernel(uint32_t *__restrict__ hashes, const uint64_t *__restrict__ coun
         } else {
 #pragma unroll
             for (int s = 0; s < 16; s++) w[s] = msg[s + srow][pl.chain];
-            pair_schedule16<16>(w, kwv, sc);
+            pair_schedule<16, B1>(w, kwv, sc);
             if (!pl.a) {
 #pragma unroll
-                for (int t = 0; t < 16; t++) kw[t][pl.lane] = kwv[t];
+                for (int t = 16; t < B1; t++) kw[t - 16][pl.lane] = kwv[t - 16];
             }
         }
         __syncthreads();
-        // phase 2: rounds P1..P2-1 | W32..47
+        // phase 2: rounds P1..P2-1 | W[B1..B2)
         if (rounds) {
 #pragma unroll
-            for (int t = 16; t < 32; t++) qa[t - 16] = kw[t - 16][pl.lane];
+            for (int t = 16; t < B1; t++) q[t - 16] = kw[t - 16][pl.lane];
             __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the rounds
 #pragma unroll
             for (int t = P1; t < P2; t++) pair_round(r1, r2, r3, r4, kwt(t), rc);
         } else {
-            pair_schedule16<32>(w, kwv, sc);
+            pair_schedule<B1, B2>(w, kwv, sc);
             if (!pl.a) {
 #pragma unroll
-                for (int t = 0; t < 16; t++) kw[t + 16][pl.lane] = kwv[t];
+                for (int t = B1; t < B2; t++) kw[t - 16][pl.lane] = kwv[t - 16];
             }
         }
         __syncthreads();
-        // phase 3: rounds P2..P3-1 | W48..63
+        // phase 3: rounds P2..P3-1 | W[B2..64)
         if (rounds) {
 #pragma unroll
-            for (int t = 32; t < 48; t++) qb[t - 32] = kw[t - 16][pl.lane];
+            for (int t = B1; t < B2; t++) q[t - 16] = kw[t - 16][pl.lane];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = P2; t < P3; t++) pair_round(r1, r2, r3, r4, kwt(t), rc);
         } else {
-            pair_schedule16<48>(w, kwv, sc);
+            pair_schedule<B2, 64>(w, kwv, sc);
             if (!pl.a) {
 #pragma unroll
-                for (int t = 0; t < 16; t++) kw[t + 32][pl.lane] = kwv[t];
+                for (int t = B2; t < 64; t++) kw[t - 16][pl.lane] = kwv[t - 16];
             }
         }
         __syncthreads();
         // phase 4: rounds P3..63, feed-forward, next message
         if (rounds) {
 #pragma unroll
-            for (int t = 48; t < 64; t++) qc[t - 48] = kw[t - 16][pl.lane];
+            for (int t = B2; t < 64; t++) q[t - 16] = kw[t - 16][pl.lane];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = P3; t < 64; t++) pair_round(r1, r2, r3, r4, kwt(t), rc);
