@@ -344,10 +344,11 @@ def test_config5_served_mix(gpu_lib, oracle):
     """The served mixed load at its stated shape: Poisson verify requests of
     log-uniform 1-2048 packets in the validator layout (1-4 signatures per
     packet, 5 % flipped, 1 % S + L, 1 % small-order keys) and PoH requests of
-    log-uniform 1-512 chains x 12,500 hashes, at two offered loads up to
-    2e6 signatures/s, through one server with its default shape (1 GiB ring).
-    Every result equals the oracle's, and equals the same request alone
-    through the drop-in ed25519_verify_many / poh_verify_many."""
+    log-uniform 1-512 chains x 12,500 hashes, at offered loads of 1e5, 2e6
+    and 1.6e7 signatures/s (the last near the load where verify p99 reaches
+    2 ms), through one server with its default shape (1 GiB ring).  Every
+    result equals the oracle's, and equals the same request alone through the
+    drop-in ed25519_verify_many / poh_verify_many."""
     def expected(b, poh_in, counts):
         mask = oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off, threads=8)
         out = poh_in.copy()
@@ -362,11 +363,11 @@ def test_config5_served_mix(gpu_lib, oracle):
     server = _server(gpu_lib)
     steps = []
     try:
-        for rate, seconds in ((1e5, 1.0), (2e6, 0.5)):
+        for rate, seconds, min_poh in ((1e5, 1.0, 5), (2e6, 0.5, 5), (1.6e7, 0.25, 1)):
             trace = loadgen.make_trace(pools, rng, rate, seconds, poh_rate=20, max_packets=2048,
                                        max_chains=512)
             inputs = [req[0].copy() if kind == "poh" else None for _, kind, req in trace]
-            _step(f"offered {rate:.0e} sig/s: {len(trace)} requests")
+            _step(f"offered {rate:.1e} sig/s: {len(trace)} requests")
             try:
                 recs, span, _ = loadgen.run_served(server, trace, wait_ms=WAIT_MS,
                                                    keep_results=True)
@@ -374,9 +375,9 @@ def test_config5_served_mix(gpu_lib, oracle):
                 server.abandon()
                 pytest.fail(str(exc))
             s = loadgen.summarize(recs, span)
-            print(f"offered {rate:.0e}:", s, server.stats(reset=True), flush=True)
+            print(f"offered {rate:.1e}:", s, server.stats(reset=True), flush=True)
             assert s["verify"]["all_exact"] and s["poh"]["all_exact"], s
-            assert s["verify"]["requests"] >= 20 and s["poh"]["requests"] >= 5
+            assert s["verify"]["requests"] >= 20 and s["poh"]["requests"] >= min_poh
             steps.append((recs, inputs))
         # chains whose counts straddle the 256-hash launch slices, and a
         # signature whose message runs past its request's packets (rejected
