@@ -94,6 +94,52 @@ def poh_issued_per_hash(kernel):
     return None
 
 
+VERIFY_FORMS = ("one-shot W=5 scalars=half waves_per_eu=1",
+                "one-shot W=5 scalars=half lanes=2 waves_per_eu=1",
+                "one-shot W=5 scalars=half staged")
+
+
+def forms_leg(lib, b, sizes=(1024, 4096, 16384), reps=3):
+    """The one-shot verify forms for batches that fit one wave per SIMD (one
+    lane, two lanes per signature, staged) on the first n signatures of the
+    validator-shaped batch, each pinned in a verify engine built for the whole
+    batch: best of `reps` launches (HIP events on the engine's stream) after
+    one untimed launch, and each mask compared with the one-lane form's."""
+    import hip_rt
+    import perf_libs
+    eng = perf_libs.VerifyEngine(lib, b.n, b.packets.size)
+    try:
+        hip_rt.upload_to(eng.d_packets, b.packets)
+        for dptr, tab in ((eng.d_lens, b.lens), (eng.d_pk, b.pk_off), (eng.d_sig, b.sig_off),
+                          (eng.d_msg, b.msg_off)):
+            hip_rt.upload_to(dptr, tab)
+        names = eng.candidates()
+        out = {}
+        for n in sizes:
+            row, ref = {}, None
+            for form in VERIFY_FORMS:
+                eng.use(names.index(form))
+                assert eng.run(n) == 0 and eng.sync() == 0
+                mask = hip_rt.download_from(eng.d_out, n)
+                ref = mask if ref is None else ref
+                best = None
+                for _ in range(reps):
+                    e0, e1 = hip_rt.Event(), hip_rt.Event()
+                    e0.record(eng.stream)
+                    eng.run(n)
+                    e1.record(eng.stream)
+                    e1.synchronize()
+                    ms = e0.elapsed_ms(e1)
+                    best = ms if best is None else min(best, ms)
+                row[form] = {"ms": best, "mask_equal": bool(np.array_equal(mask, ref))}
+            out[str(n)] = row
+            log(f"bench: forms at {n}: " + ", ".join(f"{f.split('half ')[1]} {v['ms']:.3f} ms"
+                                                      for f, v in row.items()))
+        return out
+    finally:
+        eng.close()
+
+
 POH_WORKLOADS = (("c2_4096x12500", 4096, 12500), ("sat_262144x500", 262144, 500))
 
 
@@ -574,6 +620,8 @@ def main():
                     help="with --served-only: the requests' packets in pinned host memory")
     ap.add_argument("--poh-only", action="store_true",
                     help="run only the PoH leg and print its figures")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="run only the one-shot verify forms on 1,024-16,384 signatures")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before timing "
                          "(a known-size dispatch that calibrates HBM byte counters)")
@@ -596,6 +644,9 @@ def main():
 
     log("bench: generating validator-shaped batch")
     b = W.verify_validator(lib.sign, lib.create_keypair, seed=3)
+    if args.forms_only:
+        print(json.dumps({"verify_forms": forms_leg(lib, b)}), flush=True)
+        return
     slots = tuple(int(x) for x in args.served_slots.split(","))
     if args.served_only:
         mask = lib.verify_many(b.batches, b.message_size, b.lens, b.pk_off, b.sig_off, b.msg_off)
