@@ -96,15 +96,17 @@ def poh_issued_per_hash(kernel):
 
 VERIFY_FORMS = ("one-shot W=5 scalars=half waves_per_eu=1",
                 "one-shot W=5 scalars=half lanes=2 waves_per_eu=1",
-                "one-shot W=5 scalars=half staged")
+                "one-shot W=5 scalars=half staged",
+                "one-shot W=5 scalars=half shared-hash")
 
 
-def forms_leg(lib, b, sizes=(1024, 4096, 16384), reps=3):
-    """The one-shot verify forms for batches that fit one wave per SIMD (one
-    lane, two lanes per signature, staged) on the first n signatures of the
-    validator-shaped batch, each pinned in a verify engine built for the whole
-    batch: best of `reps` launches (HIP events on the engine's stream) after
-    one untimed launch, and each mask compared with the one-lane form's."""
+def forms_leg(lib, b, sizes=(1024, 4096, 16384, None), reps=3):
+    """The one-shot verify forms with one wave per SIMD (one lane, two lanes
+    per signature, staged, shared-hash) on the first n signatures of the
+    validator-shaped batch (None: all of it), each pinned in a verify engine
+    built for the whole batch: best of `reps` launches (HIP events on the
+    engine's stream) after one untimed launch, and each mask compared with the
+    one-lane form's."""
     import hip_rt
     import perf_libs
     eng = perf_libs.VerifyEngine(lib, b.n, b.packets.size)
@@ -116,6 +118,7 @@ def forms_leg(lib, b, sizes=(1024, 4096, 16384), reps=3):
         names = eng.candidates()
         out = {}
         for n in sizes:
+            n = b.n if n is None else n
             row, ref = {}, None
             for form in VERIFY_FORMS:
                 eng.use(names.index(form))
@@ -621,7 +624,8 @@ def main():
     ap.add_argument("--poh-only", action="store_true",
                     help="run only the PoH leg and print its figures")
     ap.add_argument("--forms-only", action="store_true",
-                    help="run only the one-shot verify forms on 1,024-16,384 signatures")
+                    help="run only the one-shot verify forms on 1,024-16,384 signatures "
+                         "and the whole validator-shaped batch")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before timing "
                          "(a known-size dispatch that calibrates HBM byte counters)")

@@ -176,20 +176,23 @@ __device__ __forceinline__ uint32_t msg_window_word(const uint32_t d[33], int i,
 }
 #endif
 
-// SHA-512 over prefix (npw little-endian words, npw even) || M.
-SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw,
-                            const MsgReader &msg) {
-    sha512_init(st);
+// SHA-512 over prefix (npw little-endian words, npw even) || M: blocks
+// [b0, b1) of the padded stream (b1 clamped to its block count), continuing
+// from st (which holds the initial state when b0 = 0).  Splitting a hash at a
+// block boundary between two calls gives the same state as one call.
+SPL_HD void sha512_prefixed_blocks(uint64_t st[8], const uint32_t *prefix, uint32_t npw,
+                                   const MsgReader &msg, uint32_t b0, uint32_t b1) {
     uint64_t total = 4ull * npw + msg.len;
     uint32_t nblocks = (uint32_t)((total + 17 + 127) / 128);
+    if (b1 > nblocks) b1 = nblocks;
 #if defined(__HIP_DEVICE_COMPILE__)
     // Block b holds message words 32 b - npw .. + 31.  Each block's message
     // dwords are loaded one block ahead, while the previous block compresses,
     // instead of one dependent load (and wait) per word.
     const uint32_t sh = (uint32_t)((uintptr_t)msg.p & 3);
     uint32_t win[33];
-    msg_window(win, msg, -(int)npw);
-    for (uint32_t b = 0; b < nblocks; b++) {
+    msg_window(win, msg, 32 * (int)b0 - (int)npw);
+    for (uint32_t b = b0; b < b1; b++) {
         const int q0 = 32 * (int)b - (int)npw;
         uint32_t nxt[33];
         msg_window(nxt, msg, q0 + 32);
@@ -215,7 +218,7 @@ SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw
     }
     return;
 #endif
-    for (uint32_t b = 0; b < nblocks; b++) {
+    for (uint32_t b = b0; b < b1; b++) {
         uint64_t w[16];
         for (int t = 0; t < 16; t++) {
             uint32_t k = 32u * b + 2u * t;   // stream word index
@@ -229,6 +232,13 @@ SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw
         }
         sha512_block(st, w);
     }
+}
+
+// SHA-512 over prefix (npw little-endian words, npw even) || M.
+SPL_HD void sha512_prefixed(uint64_t st[8], const uint32_t *prefix, uint32_t npw,
+                            const MsgReader &msg) {
+    sha512_init(st);
+    sha512_prefixed_blocks(st, prefix, npw, msg, 0u, ~0u);
 }
 
 // ------------------------------------------------------------ verify

@@ -23,23 +23,34 @@ struct VerifyConfig {
                           //    y(-R) chain, each with half of the comb digits;
                           //    3 (half only, one-shot only): the staged form, the
                           //    two chain lanes in one wave and the hashing, the
-                          //    lattice step and the comb in a second wave
+                          //    lattice step and the comb in a second wave;
+                          //    4 (half only, one-shot only, W = 5 only): the
+                          //    shared-hash form, three one-lane point waves and
+                          //    a wave that starts their hashes and sums their
+                          //    combs, 192 signatures per workgroup
 };
 
 constexpr VerifyConfig kDefaultVerifyConfig = {0, 4, 1, 0, 0, 1, 1};
 constexpr VerifyConfig kPairVerifyConfig = {0, 4, 1, 0, 0, 1, 2};
 constexpr VerifyConfig kStagedVerifyConfig = {0, 5, 1, 0, 0, 1, 3};
 constexpr VerifyConfig kStagedVerifyConfigW4 = {0, 4, 1, 0, 0, 1, 3};
+constexpr VerifyConfig kSharedHashVerifyConfig = {0, 5, 1, 0, 0, 1, 4};
 
 // The untuned one-shot choice for n signatures on a device with `simds`
 // SIMDs: the staged form while its two waves per 32 signatures get a SIMD
-// each, two lanes per signature while those waves do, else one lane.
-// Measured on 512-16,384 validator-shaped signatures (256 CUs): staged
-// 0.48-0.49 ms, two lanes 0.64-0.66 ms, one lane 0.88-0.89 ms per launch.
+// each, two lanes per signature while those waves do; above that, the
+// shared-hash form (four waves per 192 signatures) or one lane per signature,
+// whichever the launch times below give fewer milliseconds for in whole
+// rounds of the device's SIMDs.  Measured per launch on validator-shaped
+// signatures (256 CUs): staged 0.48-0.51 ms, two lanes 0.64-0.66 ms,
+// shared-hash 0.75 ms, one lane 0.87-0.88 ms.
 inline VerifyConfig untuned_plain_config(uint32_t n, uint32_t simds) {
     if (16ull * simds >= n) return kStagedVerifyConfig;
     if (32ull * simds >= n) return kPairVerifyConfig;
-    return kDefaultVerifyConfig;
+    const uint64_t shared_rounds = (n + 48ull * simds - 1) / (48ull * simds);
+    const uint64_t plain_rounds = (n + 64ull * simds - 1) / (64ull * simds);
+    return 75 * shared_rounds <= 88 * plain_rounds ? kSharedHashVerifyConfig
+                                                   : kDefaultVerifyConfig;
 }
 
 // Device-side description of one verify batch.  All pointers are device

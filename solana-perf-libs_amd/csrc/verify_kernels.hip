@@ -389,6 +389,208 @@ verify_staged_kernel(VerifyJob job, uint32_t n_arg) {
     if (valid && !role) job.out[i] = (uint8_t)(id & (uint32_t)dec & (flags & 1u));
 }
 
+// Shared-hash form, for batches too large for the staged form: a workgroup
+// covers 192 signatures with four waves, each on a SIMD of its own (one
+// workgroup per CU):
+//   point waves 0-2 (one lane per signature, 64 each): the byte checks, the
+//              lockstep decode of -A and -R and both tables of multiples;
+//              then the rest of the SHA-512 after the hash wave's first KH
+//              blocks, h mod L, the half scalars and sp = y S mod L; then the
+//              doubling chain [|x|](+-A) + [y](-R) with no comb additions;
+//   hash wave  (lane j: signatures j, 64 + j and 128 + j of the group): the
+//              first KH SHA-512 blocks of its three messages while the point
+//              waves decode; then, once they have sp, the three comb sums
+//              [sp]B while they run their chains.
+// Three barriers order the handovers through LDS (48 KB per group): the
+// hash states, sp, the comb sums.  Every lane of every wave reaches all three
+// (out-of-range and rejected signatures compute on and write nothing), and a
+// group wholly past the batch leaves before the first.  A negative x flips the
+// sign of each -A table entry as it is added, so the tables need not wait for
+// the hash.  KH = 8 balances the two: on 41,626 validator-shaped signatures
+// (messages of 100-1200 bytes, 2-11 blocks) one launch took 0.771, 0.756,
+// 0.753, 0.774 and 0.789 ms with KH = 6, 7, 8, 9 and 11, against 0.882 ms
+// for one lane per signature.
+constexpr int kSharedHashBlocks = 8;
+
+struct SharedHashShared {
+    uint32_t st[16][192];   // SHA-512 state after KH blocks, as 32-bit halves
+    uint32_t sp[8][192];    // y S mod L
+    int32_t comb[40][192];  // [sp]B as a ge_cached, by limb
+};
+
+// [|x|](P1 or -P1) + [y](Pr) from tables of P1 and Pr multiples, K windows of
+// W bits, the same K in every lane of the wave.
+template <int W>
+__device__ __forceinline__ void verify_chain_half(ge_p3 &u, const uint32_t x[8],
+                                                  const uint32_t y[8], int K, const ge_cached *T1,
+                                                  const ge_cached *Tr, uint32_t flip) {
+    uint32_t a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) { a[k] = x[k]; b[k] = y[k]; }
+    shl256(a, 256 - K * W);
+    shl256(b, 256 - K * W);
+    ge_p1p1 t;
+    ge_p2 r;
+    ge_p2_0(r);
+    for (int j = K - 1; j >= 0; j--) {
+        const int d1 = take_top_digit<W>(a);
+        const int d2 = take_top_digit<W>(b);
+        if (j != K - 1) {
+#pragma unroll
+            for (int k = 0; k < W - 1; k++) {
+                ge_p2_dbl(t, r);
+                ge_p1p1_to_p2(r, t);
+            }
+            ge_p2_dbl(t, r);
+            ge_p1p1_to_p3(u, t);
+        } else {
+            ge_p3_0(u);
+        }
+        ge_add_cached(t, u, T1[d1 < 0 ? -d1 : d1], (uint32_t)(d1 < 0) ^ flip);
+        ge_p1p1_to_p3(u, t);
+        ge_add_cached(t, u, Tr[d2 < 0 ? -d2 : d2], d2 < 0);
+        if (j != 0) ge_p1p1_to_p2(r, t);
+    }
+    ge_p1p1_to_p3(u, t);
+}
+
+template <int W, int KH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+verify_shared_hash_kernel(VerifyJob job, uint32_t n_arg) {
+    raise_wave_priority();
+    __shared__ SharedHashShared sh;
+    const ed25519_tables &T = *job.T;
+    const uint32_t n = job.n_dev ? *job.n_dev : n_arg;
+    const uint32_t base = blockIdx.x * 192u;
+    // a graph captured for a larger capacity launches groups past the batch:
+    // all four waves of such a group leave here, before any barrier
+    if (base >= n) return;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave == 3) {
+        // ---- hash wave: the first KH blocks of three messages
+        for (int q = 0; q < 3; q++) {
+            const uint32_t j = 64u * q + lane, i = base + j;
+            uint64_t st[8];
+            sha512_init(st);
+            if (i < n) {
+                const uint64_t so = job.sig_off[i], po = job.pk_off[i], mo = job.msg_off[i],
+                               ml = job.lens[i];
+                if (so + 64 <= job.packets_size && po + 32 <= job.packets_size &&
+                    mo + ml <= job.packets_size) {
+                    uint32_t pre[16];
+                    load_words(pre, job.packets + so, 8);
+                    load_words(pre + 8, job.packets + po, 8);
+                    MsgReader msg{job.packets + mo, (uint32_t)ml};
+                    sha512_prefixed_blocks(st, pre, 16, msg, 0u, (uint32_t)KH);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                sh.st[2 * k][j] = (uint32_t)st[k];
+                sh.st[2 * k + 1][j] = (uint32_t)(st[k] >> 32);
+            }
+        }
+        __syncthreads();   // the hash states are out
+        __syncthreads();   // the point waves' sp are in
+        for (int q = 0; q < 3; q++) {
+            const uint32_t j = 64u * q + lane;
+            uint32_t sp[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) sp[k] = sh.sp[k][j];
+            ge_p3 pb;
+            ge_p3_0(pb);
+            ge_p1p1 tb;
+            uint32_t below = 0;
+            for (int m = 0; m < 32; m++) {
+                const int df = take_low_byte_digit(sp, below);
+                ge_add_precomp(tb, pb, T.comb[m][df < 0 ? -df : df], df < 0);
+                ge_p1p1_to_p3(pb, tb);
+            }
+            ge_cached cb;
+            ge_p3_to_cached(cb, pb, T.k.d2);
+            const int32_t *cw = reinterpret_cast<const int32_t *>(&cb);
+#pragma unroll
+            for (int k = 0; k < 40; k++) sh.comb[k][j] = cw[k];
+        }
+        __syncthreads();   // the comb sums are out
+        return;
+    }
+    // ---- point wave `wave`: signature j of the group, one lane each
+    constexpr int kT = VarWindow<W>::kTable;
+    const uint32_t j = 64u * wave + lane, i = base + j;
+    const bool valid = i < n;
+    // out-of-range lanes read (never write) the group's first tables
+    ge_cached *tabs = job.scratch + (size_t)(valid ? i : base) * lane_tables<W, 1>();
+    uint32_t sig[16], pk[8];
+#pragma unroll
+    for (int k = 0; k < 16; k++) sig[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) pk[k] = 0;
+    MsgReader msg{job.packets, 0};
+    uint32_t ok = 0;
+    bool in_range = false;   // the offsets lie in the packets (the hash wave hashed them)
+    if (valid) {
+        const uint64_t so = job.sig_off[i], po = job.pk_off[i], mo = job.msg_off[i],
+                       ml = job.lens[i];
+        if (so + 64 <= job.packets_size && po + 32 <= job.packets_size &&
+            mo + ml <= job.packets_size) {
+            in_range = true;
+            load_words(sig, job.packets + so, 16);
+            load_words(pk, job.packets + po, 8);
+            msg = MsgReader{job.packets + mo, (uint32_t)ml};
+            ok = !(sig[15] & 0xf0000000u) && enc_y_canonical(sig) && !y_low_order(sig) &&
+                 !y_low_order(pk);
+        }
+    }
+    {
+        ge_p3 pts[2];
+        const uint32_t *const enc[2] = {pk, sig};
+        uint32_t bad[2];
+        ge_decode_neg_n(pts, enc, T.k, bad);
+        ok &= !(bad[0] | bad[1]);
+        if (valid) {
+            build_multiples(tabs, kT, pts[0], T.k.d2);
+            build_multiples(tabs + kT, kT, pts[1], T.k.d2);
+        }
+    }
+    __syncthreads();   // the hash states are in
+    uint64_t st[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        st[k] = (uint64_t)sh.st[2 * k][j] | ((uint64_t)sh.st[2 * k + 1][j] << 32);
+    if (in_range) {
+        uint32_t pre[16];
+#pragma unroll
+        for (int k = 0; k < 8; k++) { pre[k] = sig[k]; pre[8 + k] = pk[k]; }
+        sha512_prefixed_blocks(st, pre, 16, msg, (uint32_t)KH, ~0u);
+    }
+    uint32_t hw[16], h[8], x[8], y[8], sp[8], x_neg;
+    sha512_digest_le_words(hw, st);
+    sc_reduce512(h, hw);
+    half_scalars(h, x, x_neg, y);
+    const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    sc_muladd(sp, y, sig + 8, zero);
+#pragma unroll
+    for (int k = 0; k < 8; k++) sh.sp[k][j] = sp[k];
+    __syncthreads();   // sp is out
+    const int K = wave_max7(half_windows<W>(half_bits(x, y)));
+    ge_p3 u;
+    verify_chain_half<W>(u, x, y, K, tabs, tabs + kT, x_neg);
+    __syncthreads();   // the comb sums are in
+    ge_cached cb;
+    int32_t *cw = reinterpret_cast<int32_t *>(&cb);
+#pragma unroll
+    for (int k = 0; k < 40; k++) cw[k] = sh.comb[k][j];
+    ge_p1p1 t;
+    ge_add_cached(t, u, cb, 0);
+    ge_p2 r;
+    ge_p1p1_to_p2(r, t);
+    fe dz;
+    fe_sub(dz, r.Y, r.Z);
+    const uint32_t id = fe_iszero(r.X) & fe_iszero(dz);
+    if (valid) job.out[i] = (uint8_t)(ok & id);
+}
+
 template <int W, int WPE, int HALF>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 verify_persistent_kernel(VerifyJob job) {
@@ -499,6 +701,11 @@ static void launch_staged(const VerifyJob &job, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL((verify_staged_kernel<W>), dim3((n + 31) / 32), dim3(128), 0, s, job, n);
 }
 
+static void launch_shared_hash(const VerifyJob &job, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL((verify_shared_hash_kernel<5, kSharedHashBlocks>), dim3((n + 191) / 192),
+                       dim3(256), 0, s, job, n);
+}
+
 template <int HALF>
 static void launch_plain_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint32_t n,
                              hipStream_t s) {
@@ -515,6 +722,11 @@ hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint
                              int num_cus, hipStream_t s) {
     if (n_grid == 0) return hipSuccess;
     const bool w5 = cfg.window == 5;
+    if (!cfg.persistent && cfg.lanes == 4) {
+        if (!w5) return hipErrorInvalidValue;   // built for W = 5 only
+        launch_shared_hash(job, n_grid, s);
+        return hipGetLastError();
+    }
     if (!cfg.persistent && cfg.lanes == 3) {
         w5 ? launch_staged<5>(job, n_grid, s) : launch_staged<4>(job, n_grid, s);
         return hipGetLastError();
@@ -549,7 +761,9 @@ hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint
 }
 
 void config_name(const VerifyConfig &c, char *buf, size_t len) {
-    if (c.lanes == 3)
+    if (c.lanes == 4)
+        snprintf(buf, len, "one-shot W=%u scalars=half shared-hash", c.window);
+    else if (c.lanes == 3)
         snprintf(buf, len, "one-shot W=%u scalars=half staged", c.window);
     else if (c.lanes == 2)
         snprintf(buf, len, "one-shot W=%u scalars=half lanes=2 waves_per_eu=%u", c.window,
@@ -581,6 +795,7 @@ int verify_config_candidates(VerifyConfig *out, int max) {
             if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, (uint8_t)wpe, 0, 0, 1, 2};
     for (int w = 4; w <= 5; w++)
         if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, 1, 0, 0, 1, 3};
+    if (k < max) out[k++] = kSharedHashVerifyConfig;
     return k;
 }
 

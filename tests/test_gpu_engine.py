@@ -229,9 +229,12 @@ def test_every_configuration(gpu_lib, oracle):
 
 def test_staged_form_sizes(gpu_lib, oracle):
     """The staged form (hash wave + point wave per 32 signatures, handing over
-    through LDS at two barriers) on batch sizes around its groups: a partly
-    filled last group, and groups wholly past the batch in a graph captured
-    for the engine's capacity (they leave before either barrier)."""
+    through LDS at two barriers) and the shared-hash form (three point waves +
+    a hash wave per 192 signatures, three barriers) on batch sizes around
+    their groups: a partly filled last group, and groups wholly past the batch
+    in a graph captured for the engine's capacity (they leave before any
+    barrier).  The messages (100-1200 bytes) end on both sides of the 8
+    SHA-512 blocks the shared-hash form's hash wave runs before handing over."""
     import perf_libs
     b = W.verify_validator(oracle.sign, oracle.create_keypair, seed=27, num_batches=12,
                            num_keys=64, flip_frac=0.1, noncanon_frac=0.03,
@@ -242,10 +245,15 @@ def test_staged_form_sizes(gpu_lib, oracle):
         _load(eng, b)
         names = eng.candidates()
         staged = [k for k, nm in enumerate(names) if "staged" in nm]
-        assert len(staged) == 2, names
-        for k in staged:
+        shared = [k for k, nm in enumerate(names) if "shared-hash" in nm]
+        assert len(staged) == 2 and len(shared) == 1, names
+        blocks = (81 + b.lens.astype(np.int64) + 127) // 128
+        assert (blocks <= 8).any() and (blocks > 8).any()
+        sizes = {k: (1, 2, 31, 32, 33, 64, 65, 500, b.n - 1, b.n) for k in staged}
+        sizes[shared[0]] = (1, 63, 64, 65, 191, 192, 193, 384, 385, b.n - 1, b.n)
+        for k, ns in sizes.items():
             eng.use(k)
-            for n in (1, 2, 31, 32, 33, 64, 65, 500, b.n - 1, b.n):
+            for n in ns:
                 assert eng.run(n) == 0 and eng.sync() == 0
                 got = _mask(eng, n)
                 assert np.array_equal(got, want[:n]), (names[k], n,
