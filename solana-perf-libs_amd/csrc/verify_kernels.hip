@@ -406,11 +406,13 @@ verify_staged_kernel(VerifyJob job, uint32_t n_arg) {
 // (out-of-range and rejected signatures compute on and write nothing), and a
 // group wholly past the batch leaves before the first.  A negative x flips the
 // sign of each -A table entry as it is added, so the tables need not wait for
-// the hash.  KH = 8 balances the two: on 41,626 validator-shaped signatures
+// the hash.  KH balances the two: on 41,626 validator-shaped signatures
 // (messages of 100-1200 bytes, 2-11 blocks) one launch took 0.771, 0.756,
-// 0.753, 0.774 and 0.789 ms with KH = 6, 7, 8, 9 and 11, against 0.882 ms
-// for one lane per signature.
-constexpr int kSharedHashBlocks = 8;
+// 0.753, 0.774 and 0.789 ms with W = 5 and KH = 6, 7, 8, 9 and 11, and
+// 0.757 and 0.765 ms with W = 4 (half the table building) and KH = 6 and 7,
+// against 0.882 ms for one lane per signature.
+template <int W>
+SPL_HD constexpr int shared_hash_blocks() { return W == 5 ? 8 : 6; }
 
 struct SharedHashShared {
     uint32_t st[16][192];   // SHA-512 state after KH blocks, as 32-bit halves
@@ -701,9 +703,10 @@ static void launch_staged(const VerifyJob &job, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL((verify_staged_kernel<W>), dim3((n + 31) / 32), dim3(128), 0, s, job, n);
 }
 
+template <int W>
 static void launch_shared_hash(const VerifyJob &job, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL((verify_shared_hash_kernel<5, kSharedHashBlocks>), dim3((n + 191) / 192),
-                       dim3(256), 0, s, job, n);
+    hipLaunchKernelGGL((verify_shared_hash_kernel<W, shared_hash_blocks<W>()>),
+                       dim3((n + 191) / 192), dim3(256), 0, s, job, n);
 }
 
 template <int HALF>
@@ -723,8 +726,7 @@ hipError_t launch_verify_cfg(const VerifyConfig &cfg, const VerifyJob &job, uint
     if (n_grid == 0) return hipSuccess;
     const bool w5 = cfg.window == 5;
     if (!cfg.persistent && cfg.lanes == 4) {
-        if (!w5) return hipErrorInvalidValue;   // built for W = 5 only
-        launch_shared_hash(job, n_grid, s);
+        w5 ? launch_shared_hash<5>(job, n_grid, s) : launch_shared_hash<4>(job, n_grid, s);
         return hipGetLastError();
     }
     if (!cfg.persistent && cfg.lanes == 3) {
@@ -795,7 +797,8 @@ int verify_config_candidates(VerifyConfig *out, int max) {
             if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, (uint8_t)wpe, 0, 0, 1, 2};
     for (int w = 4; w <= 5; w++)
         if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, 1, 0, 0, 1, 3};
-    if (k < max) out[k++] = kSharedHashVerifyConfig;
+    for (int w = 4; w <= 5; w++)
+        if (k < max) out[k++] = VerifyConfig{0, (uint8_t)w, 1, 0, 0, 1, 4};
     return k;
 }
 

@@ -71,12 +71,12 @@ extern "C" int ed25519_verify_many_device_noalloc(
     VerifyConfig cfg = tuned_plain_config(dev, total_signatures);
     if (verify_scratch_bytes(cfg, total_signatures, 0) > kNoAllocScratchBytes) {
         // the tuned configuration needs wider tables than the array holds:
-        // the W = 4 forms fit (staged or two lanes while their waves get a
-        // SIMD each)
+        // the W = 4 forms fit, chosen by the untuned rule's sizes
         const uint32_t simds = 4u * (uint32_t)device_cus(dev);
-        cfg = 16ull * simds >= total_signatures   ? kStagedVerifyConfigW4
-              : 32ull * simds >= total_signatures ? kPairVerifyConfig
-                                                  : kDefaultVerifyConfig;
+        cfg = 16ull * simds >= total_signatures          ? kStagedVerifyConfigW4
+              : 32ull * simds >= total_signatures        ? kPairVerifyConfig
+              : shared_hash_pays(total_signatures, simds) ? kSharedHashVerifyConfigW4
+                                                         : kDefaultVerifyConfig;
     }
     VerifyJob job = make_verify_job(d_packets, packets_size, d_message_lens, d_public_key_offsets,
                                     d_signature_offsets, d_message_start_offsets, d_out, scr,
