@@ -327,6 +327,60 @@ def test_every_configuration_torsion_fixture(gpu_lib):
     assert np.array_equal(d_o.download(), big_want)
 
 
+def test_every_configuration_message_lengths(gpu_lib, oracle):
+    """Every engine configuration against the oracle on messages whose
+    lengths sit on every SHA-512 block boundary of R || A || M (64 + len + 17
+    = 128 k, k = 1..10, and one byte either side), including the 6- and
+    8-block handovers of the shared-hash forms, and on short messages from 0
+    bytes; each message starts 0-3 bytes past a 4-byte boundary, a fifth of
+    the signatures have one bit flipped, and the last message ends at the last
+    byte of the packet buffer."""
+    import perf_libs
+    rng = np.random.default_rng(41)
+    lengths = [m for k in range(1, 11) for m in (128 * k - 82, 128 * k - 81, 128 * k - 80)]
+    lengths += [0, 1, 2, 3, 4, 5, 7, 8, 16, 31, 32, 33, 63, 64, 65, 127, 128, 129]
+    rows = [(m, sh) for m in lengths for sh in range(4)]
+    stride = 1304
+    n = len(rows)
+    buf = np.zeros((n, stride), np.uint8)
+    lens = np.zeros(n, np.uint32)
+    msg_off = np.zeros(n, np.uint32)
+    base = np.arange(n, dtype=np.uint32) * stride
+    for i, (m, sh) in enumerate(rows):
+        pk, sk = oracle.create_keypair(rng.bytes(32))
+        msg = rng.bytes(m)
+        sig = bytearray(oracle.sign(msg, pk, sk))
+        start = stride - m if i == n - 1 else 96 + sh
+        mb = bytearray(msg)
+        if rng.random() < 0.2:
+            bit = int(rng.integers(0, (64 + m) * 8))
+            if bit < 512:
+                sig[bit // 8] ^= 1 << (bit % 8)
+            else:
+                mb[bit // 8 - 64] ^= 1 << (bit % 8)
+        buf[i, :64] = np.frombuffer(bytes(sig), np.uint8)
+        buf[i, 64:96] = np.frombuffer(pk, np.uint8)
+        buf[i, start:start + m] = np.frombuffer(bytes(mb), np.uint8)
+        lens[i] = m
+        msg_off[i] = base[i] + start
+    flat = buf.reshape(-1)
+    b = W.Batch(flat, stride, [flat], lens, base + 64, base.copy(), msg_off)
+    assert int(b.msg_off[-1]) + int(b.lens[-1]) == b.packets.size
+    want = oracle.verify_many(b.packets, b.lens, b.pk_off, b.sig_off, b.msg_off)
+    assert 0 < want.sum() < want.size
+    eng = perf_libs.VerifyEngine(gpu_lib, n, b.packets.size)
+    try:
+        _load(eng, b)
+        for k, name in enumerate(eng.candidates()):
+            eng.use(k)
+            assert eng.run(n) == 0 and eng.sync() == 0
+            got = _mask(eng, n)
+            bad = np.nonzero(got != want)[0]
+            assert bad.size == 0, (name, [rows[i] for i in bad[:8]])
+    finally:
+        eng.close()
+
+
 def test_every_configuration_full_batches(gpu_lib, oracle):
     """Every candidate configuration against the oracle on full-size
     validator-shaped batches (about 41k signatures each: 5 % flipped bits,
