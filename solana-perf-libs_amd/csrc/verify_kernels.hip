@@ -1,6 +1,9 @@
 // Verify kernels: one-shot and persistent (work-queue) forms, window widths 4
 // and 5, two register-allocation targets, and the SHA-length bucketing
-// prepass.
+// prepass; two lanes per signature; and two forms that give a signature's
+// stages to different waves of a workgroup (staged: a hash wave and a point
+// wave per 32 signatures; shared-hash: three point waves and one hash/comb
+// wave per 192 signatures).
 //
 // Reference: src/cuda-ecc-ed25519/verify.cu:176-232 launches three kernels of
 // 64 threads, one thread per signature.  Here a single fused kernel checks a
