@@ -94,13 +94,19 @@ SPL_HD void carry_round(T c[10], int i) {
     else c[i + 1] += cy;
 }
 
-// Rounding offset of column i, 2^(b-1).  Column sums are started at this
-// value instead of 0 (a free initial accumulator for the multiply-add
-// chains), so the first carry out of each column needs no add of its own.
-#define SPL_COL_BIAS(i) ((int64_t)1 << (SPL_LIMB_BITS(i) - 1))
+// Rounding offsets for the column sums.  Column i's carry is rounded when the
+// column holds value + 2^(b_i - 1) as that carry is taken.  An even column
+// starts at its own offset, 2^25, plus the next (odd) column's, 2^24, times
+// 2^26: that multiple of 2^26 leaves the even limb's bits alone and arrives
+// in the odd column as +2^24 inside the carry, so odd columns start at 0.
+// (The compiler adds a column's starting value with a 64-bit add of its own,
+// so five offsets cost half the adds of ten.)
+#define SPL_COL_BIAS(i) \
+    (((i) & 1) ? (int64_t)0 : ((int64_t)1 << 25) + ((int64_t)1 << 50))
 
-// First carry of a biased column (c[i] = value + 2^(b-1)): the rounded carry
-// is c[i] >> b and the limb is (c[i] mod 2^b) - 2^(b-1), in [-2^(b-1), 2^(b-1)).
+// First carry of a biased column (c[i] = value + 2^(b-1) + a multiple of
+// 2^b that is carried on): the rounded carry is c[i] >> b and the limb is
+// (c[i] mod 2^b) - 2^(b-1), in [-2^(b-1), 2^(b-1)).
 SPL_HD void carry_biased(int64_t c[10], int i) {
     const int b = SPL_LIMB_BITS(i);
     const int64_t cy = c[i] >> b;
@@ -109,7 +115,9 @@ SPL_HD void carry_biased(int64_t c[10], int i) {
     else c[i + 1] += cy;
 }
 
-// Biased 64-bit column sums (SPL_COL_BIAS added) -> tight limbs.  One carry
+// Biased 64-bit column sums (SPL_COL_BIAS added) -> tight limbs.  The carries
+// run 0 -> 9, so each odd column has its offset (from the even column below)
+// before its own carry is taken.  One carry
 // chain 0 -> 9, each column taking its carry-in before its own carry, then
 // the wrap 9 -> 0 (times 19) and one more carry out of limb 0, which leaves
 // limb 1 at most ~2^15 above its tight range.  Eleven carries: two chains
