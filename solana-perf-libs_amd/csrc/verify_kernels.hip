@@ -239,8 +239,9 @@ verify_pair_kernel(VerifyJob job, uint32_t n_arg) {
 // hashes), and [sp]B once the chains are done.  A signature's critical path
 // becomes max(hash + lattice, decode + table) + chain, against their sum in
 // the two-lane form, and no stage runs twice.  Every lane of both waves
-// reaches both barriers (rejected and out-of-range signatures compute on and
-// write nothing), so the barriers sit in control flow that both waves run.
+// reaches both barriers (lanes past the batch compute on and write nothing,
+// rejected signatures compute on and write 0), so the barriers sit in control
+// flow that both waves run.
 struct StagedShared {
     uint32_t x[8][32], y[8][32];   // |x| and y, by word
     uint32_t flags[32];            // bit 0: passed the byte checks, bit 1: x < 0
@@ -406,8 +407,9 @@ verify_staged_kernel(VerifyJob job, uint32_t n_arg) {
 //              [sp]B while they run their chains.
 // Three barriers order the handovers through LDS (48 KB per group): the
 // hash states, sp, the comb sums.  Every lane of every wave reaches all three
-// (out-of-range and rejected signatures compute on and write nothing), and a
-// group wholly past the batch leaves before the first.  A negative x flips the
+// (lanes past the batch compute on and write nothing, rejected signatures
+// compute on and write 0), and a group wholly past the batch leaves before
+// the first.  A negative x flips the
 // sign of each -A table entry as it is added, so the tables need not wait for
 // the hash.  KH balances the two: on 41,626 validator-shaped signatures
 // (messages of 100-1200 bytes, 2-11 blocks) one launch took 0.771, 0.756,
