@@ -4,6 +4,8 @@
 //
 //   half_verify_check lattice N SEED  -> prints "h x x_neg y" (hex, big-endian)
 //                                        for N scalars h (random and edge values)
+//   half_verify_check field N SEED    -> limbs of f, g, f g, f^2, 2 f^2 for N
+//                                        pairs of loose field elements
 //   half_verify_check verify N SEED   -> for N signatures (valid, corrupted, with
 //                                        small-order and mixed-torsion keys and R),
 //                                        compares verify_half_one<4>/<5> with
@@ -146,11 +148,51 @@ static int run_verify(int n) {
     return diff ? 1 : 0;
 }
 
+// Field products on loose inputs: limbs up to k times their tight range
+// (k = 1, 2, 3; 3 is the most the multiply's 32-bit operand 19 g allows),
+// random and at the extremes.  Prints, per case, the limbs of f, g,
+// f g, f^2 and 2 f^2 (ten signed integers each).
+static void rand_fe(fe &f, int kind) {
+    const int k = 1 + kind % 3;
+    for (int i = 0; i < 10; i++) {
+        const int64_t m = (int64_t)k << ((i & 1) ? 24 : 25);   // limb in (-m, m)
+        const uint64_t r = next_u64();
+        int64_t v;
+        switch (kind / 3) {
+        case 0: v = (int64_t)(r % (uint64_t)(2 * m - 1)) - (m - 1); break;   // random
+        case 1: v = m - 1; break;                                             // all +max
+        case 2: v = -(m - 1); break;                                          // all -max
+        case 3: v = (i & 1) ? -(m - 1) : m - 1; break;                        // alternating
+        default: v = (r & 1) ? m - 1 : -(m - 1); break;                       // random signs
+        }
+        f.v[i] = (int32_t)v;
+    }
+}
+
+static void print_fe(const fe &f) {
+    for (int i = 0; i < 10; i++) printf(" %d", f.v[i]);
+}
+
+static int run_field(int n) {
+    for (int c = 0; c < n; c++) {
+        fe f, g, m, s, s2;
+        rand_fe(f, c % 15);
+        rand_fe(g, (c / 15) % 15);
+        fe_mul(m, f, g);
+        fe_sq(s, f);
+        fe_sq2(s2, f);
+        print_fe(f); print_fe(g); print_fe(m); print_fe(s); print_fe(s2);
+        printf("\n");
+    }
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 4) return 2;
     const int n = atoi(argv[2]);
     g_state = strtoull(argv[3], nullptr, 10);
     if (!strcmp(argv[1], "lattice")) return run_lattice(n);
     if (!strcmp(argv[1], "verify")) return run_verify(n);
+    if (!strcmp(argv[1], "field")) return run_field(n);
     return 2;
 }

@@ -7,7 +7,9 @@ the CPU with the library's own headers compiled for the host
   their sizes match the lattice's (about 128 bits);
 - verify_half_one<4> and <5> give the same verdict as verify_one<4> (the path
   the GPU parity tests pin to the oracle) on valid, corrupted, small-order,
-  mixed-torsion-key and torsion-R signatures.
+  mixed-torsion-key and torsion-R signatures;
+- the field products under them are exact mod p on loose inputs up to the
+  limb sizes the 32-bit operands allow.
 """
 import os
 import subprocess
@@ -53,6 +55,37 @@ def test_half_scalars_congruence_and_size(checker):
     assert max(rnd) <= 136 and sum(rnd) / len(rnd) < 129
     # h = 0..15 are their own half scalars (x = h, y = 1)
     assert bits[:16] == [max(1, h.bit_length()) for h in range(16)]
+
+
+P = 2**255 - 19
+LIMB_SHIFT = [0, 26, 51, 77, 102, 128, 153, 179, 204, 230]   # radix 2^25.5
+
+
+def test_field_products_on_loose_limbs(checker):
+    """fe_mul, fe_sq and fe_sq2 (fe25519.h, compiled for the host from the
+    same source as the kernels) against Python integers mod p, on inputs
+    whose limbs reach 1, 2 and 3 times their tight range, at random and at
+    every extreme (all +max, all -max, alternating, random signs): the column
+    sums with their rounding offsets must not overflow, and every output limb
+    must come back to its tight range (limb 1 at most 2^15 over)."""
+    out = subprocess.run([checker, "field", "2250", "5"], capture_output=True, text=True,
+                         check=True).stdout
+    rows = [list(map(int, l.split())) for l in out.splitlines()]
+    assert len(rows) == 2250
+
+    def val(l):
+        return sum(v << s for v, s in zip(l, LIMB_SHIFT))
+
+    for r in rows:
+        f, g, m, s, s2 = (r[10 * k:10 * k + 10] for k in range(5))
+        vf, vg = val(f), val(g)
+        assert (val(m) - vf * vg) % P == 0, (f, g)
+        assert (val(s) - vf * vf) % P == 0, f
+        assert (val(s2) - 2 * vf * vf) % P == 0, f
+        for out_limbs in (m, s, s2):
+            for i, v in enumerate(out_limbs):
+                bound = (1 << (25 if i % 2 == 0 else 24)) + ((1 << 15) if i == 1 else 0)
+                assert -bound <= v <= bound, (i, v, f, g)
 
 
 def test_half_verify_matches_full_verify(checker):
